@@ -1,0 +1,397 @@
+// api.hip - the C ABI of include/cipher_amd.h.
+//
+// Host side of the drop-in: validates and uploads the configuration tables
+// (the AppCtx fields RHSFunction reads, src/typedef.h:145-319), builds the
+// per-cell face lists from a flattened mesh (src/cipher.c:1187-1231), owns
+// every scratch buffer, and launches the kernels on the context's stream.
+// No entry point allocates after cipher_upload_mesh.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+using namespace cipher;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(CIPHER_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+struct cipher_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int precision = CIPHER_FP64;
+    cipher_layout L{};
+    DevCfg dc{};
+    std::vector<void *> allocs;   // config tables
+    // mesh
+    long long ncells = 0;
+    int32_t *cell_off = nullptr, *adj = nullptr, *bcell_off = nullptr, *bface_id = nullptr;
+    double *cellgeom = nullptr;
+    void *cm = nullptr;
+    int32_t *glist = nullptr;
+    int32_t *err = nullptr;
+    double *partial = nullptr;
+    double *norm = nullptr;
+    double *h_norm = nullptr;
+    int32_t *h_err = nullptr;
+    int npart = 1024;
+};
+
+static void free_mesh(cipher_ctx *c) {
+    void *ptrs[] = {c->cell_off, c->adj, c->bcell_off, c->bface_id, c->cellgeom, c->cm, c->glist};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    c->cell_off = c->adj = c->bcell_off = c->bface_id = nullptr;
+    c->cellgeom = nullptr;
+    c->cm = nullptr;
+    c->glist = nullptr;
+    c->ncells = 0;
+}
+
+extern "C" const char *cipher_last_error(void) { return g_err.c_str(); }
+
+extern "C" int cipher_layout_of(const cipher_config *cfg, cipher_layout *out) {
+    if (!cfg || !out) return fail(CIPHER_ERR_ARG, "null argument");
+    if (cfg->nmat <= 0 || !cfg->material) return fail(CIPHER_ERR_ARG, "no materials");
+    int mpn = cfg->max_phase_neighbours > 0 ? cfg->max_phase_neighbours : CIPHER_MAXAP;
+    cipher_layout L{};
+    L.maxsites = 0;
+    for (int m = 0; m < cfg->nmat; m++) L.maxsites = std::max(L.maxsites, (int32_t)cfg->material[m].nsites);
+    L.sf_size = L.maxsites * cfg->ncp;
+    L.sp_size = L.maxsites * (cfg->ncp - 1);
+    L.as_offset = 0;
+    L.as_size = std::min(mpn, (int)cfg->npf) + 1;
+    L.pf_offset = L.as_offset + L.as_size;
+    L.pf_size = std::min(mpn, (int)cfg->npf);
+    L.dp_offset = L.pf_offset + L.pf_size;
+    L.dp_size = cfg->ncp - 1;
+    L.ex_offset = L.dp_offset + L.dp_size;
+    L.ex_size = L.pf_size * L.sp_size;
+    L.tm_offset = L.ex_offset + L.ex_size;
+    L.tm_size = 1;
+    L.ndof = L.tm_offset + L.tm_size;
+    *out = L;
+    return CIPHER_OK;
+}
+
+template <class T>
+static int upload(cipher_ctx *c, const T *src, int n, const T **dst) {
+    size_t bytes = sizeof(T) * (size_t)std::max(n, 1);
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, bytes));
+    c->allocs.push_back(p);
+    if (n > 0) HIPCHK(hipMemcpy(p, src, sizeof(T) * (size_t)n, hipMemcpyHostToDevice));
+    else HIPCHK(hipMemset(p, 0, bytes));
+    *dst = (const T *)p;
+    return CIPHER_OK;
+}
+
+static int validate(const cipher_config *cfg) {
+    if (cfg->dim < 1 || cfg->dim > 3) return fail(CIPHER_ERR_ARG, "dim must be 1..3");
+    if (cfg->ncp < 1 || cfg->ncp > CIPHER_MAXCP) return fail(CIPHER_ERR_ARG, "ncp must be 1..5");
+    if (cfg->npf < 1) return fail(CIPHER_ERR_ARG, "npf must be positive");
+    if (cfg->npf > 65535) return fail(CIPHER_ERR_ARG, "phase ids are uint16 (npf <= 65535)");
+    if (cfg->nf < 1 || !cfg->interface) return fail(CIPHER_ERR_ARG, "no interfaces");
+    if (!cfg->phase_material || !cfg->interface_list) return fail(CIPHER_ERR_ARG, "missing mappings");
+    if (cfg->gradient_calculation)
+        return fail(CIPHER_ERR_UNSUPPORTED, "gradient_calculation (anisotropy) is not on the GPU path");
+    if (cfg->interpolation < CIPHER_INTERP_LINEAR || cfg->interpolation > CIPHER_INTERP_CUBIC)
+        return fail(CIPHER_ERR_ARG, "interpolation");
+    if (cfg->ntseries < 1 || !cfg->tseries) return fail(CIPHER_ERR_ARG, "tseries pool must hold the empty series");
+    for (int i = 0; i < cfg->ntseries; i++)
+        if (cfg->tseries[i].n < 0 || cfg->tseries[i].n > CIPHER_MAXTSER) return fail(CIPHER_ERR_ARG, "tseries size");
+    for (int i = 0; i < cfg->nrk; i++) {
+        const cipher_rk &r = cfg->rk[i];
+        if (r.n < 0 || (r.n > 0 && (r.ts < 0 || r.ts + r.n > cfg->ntseries)))
+            return fail(CIPHER_ERR_ARG, "rk table out of range");
+    }
+    for (int p = 0; p < cfg->npf; p++)
+        if (cfg->phase_material[p] < 0 || cfg->phase_material[p] >= cfg->nmat)
+            return fail(CIPHER_ERR_ARG, "phase_material out of range");
+    for (int i = 0; i < cfg->npf; i++)
+        for (int j = i + 1; j < cfg->npf; j++) {
+            int it = cfg->interface_list[(size_t)i * cfg->npf + j];
+            if (it < 0 || it >= cfg->nf) return fail(CIPHER_ERR_ARG, "interface_list out of range");
+        }
+    auto ts_ok = [&](int i) { return i >= 0 && i < cfg->ntseries; };
+    for (int m = 0; m < cfg->nmat; m++) {
+        const cipher_material &mt = cfg->material[m];
+        if (!(mt.model >= CIPHER_CHEM_QUADRATIC && mt.model <= CIPHER_CHEM_NONE))
+            return fail(CIPHER_ERR_ARG, "material model");
+        if (!ts_ok(mt.ref) || !ts_ok(mt.specific_heat) || !ts_ok(mt.latent_heat) || !ts_ok(mt.tconductivity))
+            return fail(CIPHER_ERR_ARG, "material tseries index");
+        if (mt.molarvolume == 0.0) return fail(CIPHER_ERR_ARG, "molarvolume must be nonzero");
+        if (mt.model != CIPHER_CHEM_NONE && (mt.mobility < 0 || mt.mobility + cfg->ncp > cfg->nmobility))
+            return fail(CIPHER_ERR_ARG, "material mobility index");
+    }
+    for (int i = 0; i < cfg->nf; i++) {
+        const cipher_interface &f = cfg->interface[i];
+        if (!ts_ok(f.e_act) || !ts_ok(f.m_act)) return fail(CIPHER_ERR_ARG, "interface tseries index");
+        if (f.width == 0.0) return fail(CIPHER_ERR_ARG, "interface width must be nonzero");
+    }
+    return CIPHER_OK;
+}
+
+extern "C" int cipher_ctx_create(const cipher_config *cfg, int precision, cipher_ctx **out) {
+    if (!cfg || !out) return fail(CIPHER_ERR_ARG, "null argument");
+    if (precision != CIPHER_FP64 && precision != CIPHER_FP32_MIXED) return fail(CIPHER_ERR_ARG, "precision");
+    int rc = validate(cfg);
+    if (rc) return rc;
+    cipher_ctx *c = new cipher_ctx();
+    c->precision = precision;
+    HIPCHK(hipGetDevice(&c->device));
+    cipher_layout_of(cfg, &c->L);
+    if (c->L.pf_size > 16) {
+        delete c;
+        return fail(CIPHER_ERR_UNSUPPORTED, "active-set capacity above 16");
+    }
+    DevCfg &d = c->dc;
+    int r = 0;
+    r |= upload(c, cfg->tseries, cfg->ntseries, &d.ts);
+    r |= upload(c, cfg->rk, cfg->nrk, &d.rk);
+    r |= upload(c, cfg->rkindex, cfg->nrkindex, &d.rkindex);
+    r |= upload(c, cfg->mobility, cfg->nmobility, &d.mob);
+    r |= upload(c, cfg->material, cfg->nmat, &d.mat);
+    r |= upload(c, cfg->interface, cfg->nf, &d.itf);
+    r |= upload(c, cfg->source, cfg->nsource, &d.src);
+    r |= upload(c, cfg->bc, cfg->nbcs, &d.bc);
+    r |= upload(c, cfg->aniso, cfg->naniso, &d.aniso);
+    r |= upload(c, cfg->phase_material, cfg->npf, &d.phase_material);
+    r |= upload(c, cfg->interface_list, cfg->npf * cfg->npf, &d.ilist);
+    if (r) {
+        cipher_ctx_destroy(c);
+        return CIPHER_ERR_HIP;
+    }
+    d.junctionpenalty = cfg->junctionpenalty;
+    d.trate = cfg->temperature_rate;
+    d.has_trate = cfg->has_temperature_rate;
+    d.dim = cfg->dim;
+    d.npf = cfg->npf;
+    d.ncp = cfg->ncp;
+    d.nd = cfg->ncp - 1;
+    d.nmat = cfg->nmat;
+    d.nf = cfg->nf;
+    d.nbcs = cfg->nbcs;
+    d.interp = cfg->interpolation;
+    d.P = c->L.pf_size;
+    d.D = c->L.ndof;
+    d.aso = c->L.as_offset;
+    d.pfo = c->L.pf_offset;
+    d.dpo = c->L.dp_offset;
+    d.exo = c->L.ex_offset;
+    d.tmo = c->L.tm_offset;
+    d.SP = c->L.sp_size;
+    d.SF = c->L.sf_size;
+    d.has_tcond = d.has_theat = d.has_sources = d.has_potential = d.has_imobc = d.has_iact = 0;
+    for (int m = 0; m < cfg->nmat; m++) {
+        const cipher_material &mt = cfg->material[m];
+        if (cfg->tseries[mt.tconductivity].n) d.has_tcond = 1;
+        if (cfg->tseries[mt.specific_heat].n || cfg->tseries[mt.latent_heat].n) d.has_theat = 1;
+        for (int s = 0; s < mt.nsources; s++)
+            if (cfg->source[mt.source + s].model == CIPHER_SOURCE_SINK) d.has_sources = 1;
+    }
+    for (int i = 0; i < cfg->nf; i++) {
+        const cipher_interface &f = cfg->interface[i];
+        if (f.has_potential) d.has_potential = 1;
+        if (f.has_mobilityc) d.has_imobc = 1;
+        if (cfg->tseries[f.e_act].n || cfg->tseries[f.m_act].n) d.has_iact = 1;
+        for (int s = 0; s < f.nsources; s++)
+            if (cfg->source[f.source + s].model == CIPHER_SOURCE_SINK) d.has_sources = 1;
+    }
+    if (hipMalloc(&c->err, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->partial, sizeof(double) * c->npart) != hipSuccess ||
+        hipMalloc(&c->norm, sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&c->h_norm, sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&c->h_err, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+        cipher_ctx_destroy(c);
+        return fail(CIPHER_ERR_HIP, "scratch allocation failed");
+    }
+    *out = c;
+    return CIPHER_OK;
+}
+
+extern "C" int cipher_ctx_destroy(cipher_ctx *c) {
+    if (!c) return CIPHER_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    else (void)hipDeviceSynchronize();
+    free_mesh(c);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->err) (void)hipFree(c->err);
+    if (c->partial) (void)hipFree(c->partial);
+    if (c->norm) (void)hipFree(c->norm);
+    if (c->h_norm) (void)hipHostFree(c->h_norm);
+    if (c->h_err) (void)hipHostFree(c->h_err);
+    delete c;
+    return CIPHER_OK;
+}
+
+extern "C" int cipher_ctx_layout(const cipher_ctx *c, cipher_layout *out) {
+    if (!c || !out) return fail(CIPHER_ERR_ARG, "null argument");
+    *out = c->L;
+    return CIPHER_OK;
+}
+
+extern "C" int cipher_set_precision(cipher_ctx *c, int precision) {
+    if (!c) return fail(CIPHER_ERR_ARG, "null ctx");
+    if (precision != CIPHER_FP64 && precision != CIPHER_FP32_MIXED) return fail(CIPHER_ERR_ARG, "precision");
+    c->precision = precision;
+    return CIPHER_OK;
+}
+
+extern "C" int cipher_set_stream(cipher_ctx *c, void *stream) {
+    if (!c) return fail(CIPHER_ERR_ARG, "null ctx");
+    c->stream = (hipStream_t)stream;
+    return CIPHER_OK;
+}
+
+extern "C" int cipher_set_temperature_rate(cipher_ctx *c, int has_rate, double rate) {
+    if (!c) return fail(CIPHER_ERR_ARG, "null ctx");
+    c->dc.has_trate = has_rate ? 1 : 0;
+    c->dc.trate = rate;
+    return CIPHER_OK;
+}
+
+extern "C" int cipher_upload_mesh(cipher_ctx *c, const cipher_mesh *m) {
+    if (!c || !m) return fail(CIPHER_ERR_ARG, "null argument");
+    if (m->ncells < 0 || m->nfaces < 0 || m->nbfaces < 0) return fail(CIPHER_ERR_ARG, "negative sizes");
+    if (m->ncells >= (1 << 30)) return fail(CIPHER_ERR_ARG, "more than 2^30 cells");
+    if ((m->nfaces && !m->face_cells) || (m->ncells && !m->cellgeom) ||
+        (m->nbfaces && (!m->bface_cell || !m->bface_id)))
+        return fail(CIPHER_ERR_ARG, "null mesh array");
+    const long long N = m->ncells;
+    std::vector<int32_t> off(N + 1, 0), boff(N + 1, 0);
+    for (int f = 0; f < m->nfaces; f++) {
+        int a = m->face_cells[2 * f], b = m->face_cells[2 * f + 1];
+        if (a < 0 || a >= N || b < 0 || b >= N) return fail(CIPHER_ERR_ARG, "face cell out of range");
+        off[a + 1]++;
+        off[b + 1]++;
+    }
+    for (long long i = 0; i < N; i++) off[i + 1] += off[i];
+    std::vector<int32_t> adj(std::max<long long>(off[N], 1));
+    std::vector<int32_t> pos(off.begin(), off.end() - 1);
+    for (int f = 0; f < m->nfaces; f++) {
+        int a = m->face_cells[2 * f], b = m->face_cells[2 * f + 1];
+        adj[pos[a]++] = (b << 1) | 0;   // a is L
+        adj[pos[b]++] = (a << 1) | 1;   // b is R
+    }
+    for (int k = 0; k < m->nbfaces; k++) {
+        int a = m->bface_cell[k];
+        if (a < 0 || a >= N) return fail(CIPHER_ERR_ARG, "boundary face cell out of range");
+        boff[a + 1]++;
+    }
+    for (long long i = 0; i < N; i++) boff[i + 1] += boff[i];
+    std::vector<int32_t> bid(std::max<long long>(boff[N], 1));
+    std::vector<int32_t> bpos(boff.begin(), boff.end() - 1);
+    for (int k = 0; k < m->nbfaces; k++) bid[bpos[m->bface_cell[k]]++] = m->bface_id[k];
+    free_mesh(c);
+    size_t cmrec = std::max(prep_record_bytes(c->dc.ncp, CIPHER_FP64), prep_record_bytes(c->dc.ncp, CIPHER_FP32_MIXED));
+    HIPCHK(hipMalloc(&c->cell_off, sizeof(int32_t) * (N + 1)));
+    HIPCHK(hipMalloc(&c->adj, sizeof(int32_t) * adj.size()));
+    HIPCHK(hipMalloc(&c->bcell_off, sizeof(int32_t) * (N + 1)));
+    HIPCHK(hipMalloc(&c->bface_id, sizeof(int32_t) * bid.size()));
+    HIPCHK(hipMalloc(&c->cellgeom, sizeof(double) * std::max<long long>(N, 1)));
+    HIPCHK(hipMalloc(&c->cm, cmrec * std::max<long long>(N, 1)));
+    HIPCHK(hipMalloc(&c->glist, sizeof(int32_t) * (size_t)(c->L.pf_size + 1) * std::max<long long>(N, 1)));
+    HIPCHK(hipMemcpy(c->cell_off, off.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->adj, adj.data(), sizeof(int32_t) * adj.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->bcell_off, boff.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->bface_id, bid.data(), sizeof(int32_t) * bid.size(), hipMemcpyHostToDevice));
+    if (N) HIPCHK(hipMemcpy(c->cellgeom, m->cellgeom, sizeof(double) * N, hipMemcpyHostToDevice));
+    c->ncells = N;
+    return CIPHER_OK;
+}
+
+static int rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active) {
+    if (!c || !X || !F) return fail(CIPHER_ERR_ARG, "null argument");
+    if (!c->cell_off) return fail(CIPHER_ERR_STATE, "no mesh uploaded");
+    RhsLaunch L;
+    L.cfg = &c->dc;
+    L.X = X;
+    L.F = F;
+    L.active = active;
+    L.cellgeom = c->cellgeom;
+    L.cell_off = c->cell_off;
+    L.adj = c->adj;
+    L.bcell_off = c->bcell_off;
+    L.bface_id = c->bface_id;
+    L.cm = c->cm;
+    L.ncells = c->ncells;
+    cipher_status s = launch_rhs(L, c->precision, c->stream);
+    if (s != CIPHER_OK) return fail(s, std::string("rhs launch: ") + hipGetErrorString(hipGetLastError()));
+    return CIPHER_OK;
+}
+
+extern "C" int cipher_rhs(cipher_ctx *c, double t, const double *X, double *F) {
+    (void)t;
+    return rhs_impl(c, X, F, nullptr);
+}
+
+extern "C" int cipher_rhs_active(cipher_ctx *c, double t, const double *X, double *F, uint32_t *active) {
+    (void)t;
+    if (!active) return fail(CIPHER_ERR_ARG, "null active buffer");
+    return rhs_impl(c, X, F, active);
+}
+
+static int post_common(cipher_ctx *c, double *X, const int32_t *phase, bool setup) {
+    if (!c || !X) return fail(CIPHER_ERR_ARG, "null argument");
+    if (!c->cell_off) return fail(CIPHER_ERR_STATE, "no mesh uploaded");
+    PostLaunch L;
+    L.cfg = &c->dc;
+    L.X = X;
+    L.cell_off = c->cell_off;
+    L.adj = c->adj;
+    L.glist = c->glist;
+    L.err = c->err;
+    L.phase = phase;
+    L.ncells = c->ncells;
+    HIPCHK(hipMemsetAsync(c->err, 0, sizeof(int32_t), c->stream));
+    if (setup) HIPCHK(hipMemsetAsync(X, 0, sizeof(double) * (size_t)c->ncells * c->L.ndof, c->stream));
+    cipher_status s = setup ? launch_setup(L, c->stream) : launch_poststep(L, c->stream);
+    if (s != CIPHER_OK) return fail(s, std::string("poststep launch: ") + hipGetErrorString(hipGetLastError()));
+    HIPCHK(hipMemcpyAsync(c->h_err, c->err, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (*c->h_err) return fail(CIPHER_ERR_CAPACITY, "active-set union exceeds the capacity PF_SIZE");
+    return CIPHER_OK;
+}
+
+extern "C" int cipher_poststep(cipher_ctx *c, double *X) { return post_common(c, X, nullptr, false); }
+
+extern "C" int cipher_setup_problem(cipher_ctx *c, const int32_t *phase, double *X) {
+    if (!phase) return fail(CIPHER_ERR_ARG, "null phase labels");
+    return post_common(c, X, phase, true);
+}
+
+extern "C" int cipher_wrms_norm(cipher_ctx *c, const double *U, const double *Y, int64_t n, double atol, double rtol,
+                                double *norm_out) {
+    if (!c || !U || !Y || !norm_out || n < 0) return fail(CIPHER_ERR_ARG, "bad argument");
+    cipher_status s = launch_wrms(U, Y, n, atol, rtol, c->partial, c->npart, c->norm, c->stream);
+    if (s != CIPHER_OK) return fail(s, "wrms launch");
+    HIPCHK(hipMemcpyAsync(c->h_norm, c->norm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *norm_out = *c->h_norm;
+    if (!std::isfinite(*norm_out)) return fail(CIPHER_ERR_NONFINITE, "error norm is not finite");
+    return CIPHER_OK;
+}
+
+extern "C" int cipher_axpy_n(cipher_ctx *c, int64_t n, double *Y, const double *X, int nk, const double *a,
+                             const double *const *K) {
+    if (!c || !Y || !X || n < 0 || nk < 0 || nk > 8 || (nk > 0 && (!a || !K)))
+        return fail(CIPHER_ERR_ARG, "bad argument");
+    cipher_status s = launch_axpy_n(n, Y, X, nk, a, K, c->stream);
+    if (s != CIPHER_OK) return fail(s, "axpy launch");
+    return CIPHER_OK;
+}

@@ -1,0 +1,107 @@
+// ts.hip - FP64 time-integration arithmetic on gfx950.
+//
+// k_wrms_partial/k_wrms_final: weighted RMS error norm of PETSc's
+//   TSErrorWeightedNorm (2-norm form, scalar atol/rtol):
+//   sqrt( (1/N) sum_i ( |u_i - y_i| / (atol + rtol*max(|u_i|,|y_i|)) )^2 ),
+//   as a deterministic two-pass reduction (fixed block partition, wavefront
+//   shuffles, fixed-order final sum) so repeated calls give identical bits.
+// k_axpy_n: Y = X + (((a0 K0 + a1 K1) + a2 K2) + ...), the stage/update
+//   combination of an explicit RK step, streamed once through HBM.
+#include "kernels.h"
+
+namespace cipher {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_wrms_partial(const double *__restrict__ U, const double *__restrict__ Y,
+                                                      long long n, double atol, double rtol, double *partial) {
+    __shared__ double red[4];
+    double s = 0.0;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double u = U[i], y = Y[i];
+        const double au = fabs(u), ay = fabs(y);
+        const double tol = atol + rtol * (au > ay ? au : ay);
+        const double e = fabs(u - y) / tol;
+        s += e * e;
+    }
+    s = wave_sum(s);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+__global__ __launch_bounds__(256) void k_wrms_final(const double *partial, int npart, long long n, double *out) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < npart; i += blockDim.x) s += partial[i];
+    s = wave_sum(s);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double tot = ((red[0] + red[1]) + red[2]) + red[3];
+        out[0] = n > 0 ? sqrt(tot / (double)n) : 0.0;
+    }
+}
+
+cipher_status launch_wrms(const double *U, const double *Y, long long n, double atol, double rtol,
+                          double *partial, int npart, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_wrms_partial, dim3(npart), dim3(256), 0, st, U, Y, n, atol, rtol, partial);
+    hipLaunchKernelGGL(k_wrms_final, dim3(1), dim3(256), 0, st, partial, npart, n, out);
+    return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
+}
+
+struct AxpyArgs {
+    const double *K[8];
+    double a[8];
+};
+
+template <int NK>
+__global__ __launch_bounds__(256) void k_axpy(long long n, double *__restrict__ Y, const double *__restrict__ X,
+                                              AxpyArgs p) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double t = 0.0;
+        if (NK > 0) {
+            t = p.a[0] * p.K[0][i];
+#pragma unroll
+            for (int j = 1; j < NK; j++) t += p.a[j] * p.K[j][i];
+            Y[i] = X[i] + t;
+        } else {
+            Y[i] = X[i];
+        }
+    }
+}
+
+cipher_status launch_axpy_n(long long n, double *Y, const double *X, int nk, const double *a,
+                            const double *const *K, hipStream_t st) {
+    if (nk < 0 || nk > 8) return CIPHER_ERR_ARG;
+    AxpyArgs p;
+    for (int j = 0; j < 8; j++) {
+        p.K[j] = j < nk ? K[j] : nullptr;
+        p.a[j] = j < nk ? a[j] : 0.0;
+    }
+    long long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    switch (nk) {
+    case 0: hipLaunchKernelGGL((k_axpy<0>), dim3((unsigned)blocks), dim3(256), 0, st, n, Y, X, p); break;
+    case 1: hipLaunchKernelGGL((k_axpy<1>), dim3((unsigned)blocks), dim3(256), 0, st, n, Y, X, p); break;
+    case 2: hipLaunchKernelGGL((k_axpy<2>), dim3((unsigned)blocks), dim3(256), 0, st, n, Y, X, p); break;
+    case 3: hipLaunchKernelGGL((k_axpy<3>), dim3((unsigned)blocks), dim3(256), 0, st, n, Y, X, p); break;
+    case 4: hipLaunchKernelGGL((k_axpy<4>), dim3((unsigned)blocks), dim3(256), 0, st, n, Y, X, p); break;
+    case 5: hipLaunchKernelGGL((k_axpy<5>), dim3((unsigned)blocks), dim3(256), 0, st, n, Y, X, p); break;
+    case 6: hipLaunchKernelGGL((k_axpy<6>), dim3((unsigned)blocks), dim3(256), 0, st, n, Y, X, p); break;
+    case 7: hipLaunchKernelGGL((k_axpy<7>), dim3((unsigned)blocks), dim3(256), 0, st, n, Y, X, p); break;
+    default: hipLaunchKernelGGL((k_axpy<8>), dim3((unsigned)blocks), dim3(256), 0, st, n, Y, X, p); break;
+    }
+    return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
+}
+
+}  // namespace cipher

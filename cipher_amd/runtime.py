@@ -1,0 +1,131 @@
+"""Python host mirror of CIPHER's right-hand-side interface over the C ABI.
+
+``Context`` owns a ``cipher_ctx`` (configuration tables + mesh lists in HBM)
+and calls the HIP library on torch-allocated device buffers, on torch's
+current stream.  ``RHSFunction`` and ``PostStep`` keep the reference's names
+and argument meaning (src/cipher.c:20-21): the solution/residual vectors are
+device tensors in the per-cell AoS layout.  There is no CPU fallback: the
+library must be built and a GPU present.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    def __init__(self, cfg, precision=abi.FP64, device=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("cipher_amd needs a GPU: no HIP device is visible")
+        self.torch = torch
+        self.lib = abi.load_library()
+        self.cfg = cfg
+        self.device = torch.device("cuda", 0 if device is None else device)
+        torch.cuda.set_device(self.device)
+        h = C.c_void_p()
+        abi.check(self.lib.cipher_ctx_create(C.byref(cfg.struct), precision, C.byref(h)), "cipher_ctx_create")
+        self.h = h
+        lay = abi.Layout()
+        abi.check(self.lib.cipher_ctx_layout(h, C.byref(lay)), "cipher_ctx_layout")
+        self.layout = {f: getattr(lay, f) for f, _ in abi.Layout._fields_}
+        self.ndof = self.layout["ndof"]
+        self.ncells = 0
+        self.precision = precision
+        self._stream()
+
+    def _stream(self):
+        s = self.torch.cuda.current_stream(self.device).cuda_stream
+        abi.check(self.lib.cipher_set_stream(self.h, C.c_void_p(s)), "cipher_set_stream")
+
+    def set_precision(self, precision):
+        abi.check(self.lib.cipher_set_precision(self.h, precision), "cipher_set_precision")
+        self.precision = precision
+
+    def upload_mesh(self, mesh):
+        ms = mesh.struct()
+        abi.check(self.lib.cipher_upload_mesh(self.h, C.byref(ms)), "cipher_upload_mesh")
+        self.ncells = mesh.ncells
+        self.mesh = mesh
+
+    def empty_state(self):
+        return self.torch.zeros(self.ncells * self.ndof, dtype=self.torch.float64, device=self.device)
+
+    def _check_state(self, X, what):
+        if X.dtype != self.torch.float64 or not X.is_cuda or not X.is_contiguous():
+            raise ValueError(f"{what}: expected a contiguous float64 device tensor")
+        if X.numel() != self.ncells * self.ndof:
+            raise ValueError(f"{what}: expected {self.ncells * self.ndof} doubles, got {X.numel()}")
+
+    def rhs(self, X, F=None, t=0.0, active=None):
+        self._check_state(X, "X")
+        if F is None:
+            F = self.torch.empty_like(X)
+        self._check_state(F, "F")
+        self._stream()
+        if active is None:
+            rc = self.lib.cipher_rhs(self.h, t, _ptr(X), _ptr(F))
+        else:
+            if active.dtype != self.torch.int32 or active.numel() != self.ncells:
+                raise ValueError("active: expected int32 tensor of ncells")
+            rc = self.lib.cipher_rhs_active(self.h, t, _ptr(X), _ptr(F), _ptr(active))
+        abi.check(rc, "cipher_rhs")
+        return F
+
+    def poststep(self, X):
+        self._check_state(X, "X")
+        self._stream()
+        abi.check(self.lib.cipher_poststep(self.h, _ptr(X)), "cipher_poststep")
+        return X
+
+    def setup_problem(self, phase):
+        ph = self.torch.as_tensor(np.ascontiguousarray(phase, dtype=np.int32)).to(self.device)
+        X = self.empty_state()
+        self._stream()
+        abi.check(self.lib.cipher_setup_problem(self.h, _ptr(ph), _ptr(X)), "cipher_setup_problem")
+        return X
+
+    def wrms_norm(self, U, Y, atol, rtol):
+        out = C.c_double()
+        self._stream()
+        abi.check(self.lib.cipher_wrms_norm(self.h, _ptr(U), _ptr(Y), U.numel(), atol, rtol, C.byref(out)),
+                  "cipher_wrms_norm")
+        return out.value
+
+    def axpy_n(self, Y, X, coeffs, Ks):
+        nk = len(coeffs)
+        a = (C.c_double * max(nk, 1))(*coeffs)
+        ks = (C.c_void_p * max(nk, 1))(*[K.data_ptr() for K in Ks])
+        self._stream()
+        abi.check(self.lib.cipher_axpy_n(self.h, X.numel(), _ptr(Y), _ptr(X), nk, a, ks), "cipher_axpy_n")
+        return Y
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cipher_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def RHSFunction(ts, ftime, X, F, user):
+    """Mirror of RHSFunction(TS,PetscReal,Vec,Vec,void*) (src/cipher.c:27):
+    user is a Context; X and F are device state tensors."""
+    user.rhs(X, F, t=ftime)
+    return 0
+
+
+def PostStep(ts):
+    """Mirror of PostStep(TS) (src/cipher.c:686), active-set block: ts must
+    expose .solution (device tensor) and .user (Context)."""
+    ts.user.poststep(ts.solution)
+    return 0

@@ -1,0 +1,260 @@
+/*
+ * cipher_amd.h - C ABI of the MI355X phase-field right-hand side.
+ *
+ * This library is a drop-in for the per-stage right-hand side of CIPHER
+ * (Calphad Integrated PHase-field solvER).  Every entry point takes plain
+ * pointers and sizes; state vectors use CIPHER's per-cell layout
+ *
+ *     [ AS | PF | DP | EX | TM ]   (doubles, D = AS+PF+DP+EX+TM per cell)
+ *
+ *   AS[0]      number n of active phases, AS[1..n] ascending phase ids
+ *              (integers stored as doubles, rounded on read)
+ *   PF[g]      phase fraction of phase AS[g+1]
+ *   DP[c]      chemical potential of component c < ncp-1
+ *   EX[g*SP + s*ndp + c]  explicit (non-ideal) site potential
+ *   TM         temperature
+ *
+ * with the sizes of src/init.c:1330-1343.  Pointers named *_dev are device
+ * pointers (HBM); every other pointer is host memory.  Calls on one context
+ * are ordered on that context's stream.  Every function returns a
+ * cipher_status; the library never aborts the caller.
+ *
+ * Reference interfaces replaced (file:line in micmog/CIPHER):
+ *   cipher_rhs            RHSFunction(TS,PetscReal,Vec,Vec,void*)   src/cipher.c:20,27-681
+ *   cipher_poststep       PostStep(TS), active-set block            src/cipher.c:21,869-974
+ *   cipher_setup_problem  SetUpProblem(Vec,AppCtx*)                 src/init.c:1721-1830
+ *   cipher_wrms_norm      TSErrorWeightedNorm (PETSc, configured at src/cipher.c:1112)
+ *   cipher_rk_*           TSRK stage arithmetic (PETSc, configured at src/cipher.c:1104)
+ *   cipher_config         AppCtx material/interface tables          src/typedef.h:145-319
+ *   cipher_mesh           flattened cells/faces/cellgeom            src/cipher.c:1187-1231
+ */
+#ifndef CIPHER_AMD_H
+#define CIPHER_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Constants of src/typedef.h:25-30. */
+#define CIPHER_LARGE 1.0e+6
+#define CIPHER_TOL 1.0e-6
+#define CIPHER_MAXCP 5
+#define CIPHER_MAXAP 15
+#define CIPHER_R_GAS_CONST 8.314462
+#define CIPHER_MAXTSER 10
+
+typedef enum {
+    CIPHER_OK = 0,
+    CIPHER_ERR_ARG = 1,         /* bad argument / unsupported combination */
+    CIPHER_ERR_HIP = 2,         /* HIP runtime error */
+    CIPHER_ERR_CAPACITY = 3,    /* active-set union exceeded capacity (src/cipher.c:968) */
+    CIPHER_ERR_NONFINITE = 4,   /* NaN/Inf in an error norm */
+    CIPHER_ERR_UNSUPPORTED = 5, /* feature not available on this path */
+    CIPHER_ERR_STATE = 6        /* call out of order (e.g. no mesh uploaded) */
+} cipher_status;
+
+/* chemfe_model_t, src/typedef.h:54-59 (same numbering) */
+enum { CIPHER_CHEM_QUADRATIC = 0, CIPHER_CHEM_CALPHADDIS = 1,
+       CIPHER_CHEM_CALPHAD2SL = 2, CIPHER_CHEM_NONE = 3 };
+/* interpolation_t, src/typedef.h:69-74 */
+enum { CIPHER_INTERP_LINEAR = 0, CIPHER_INTERP_QUADRATIC = 1,
+       CIPHER_INTERP_CUBIC = 2 };
+/* boundary_t, src/typedef.h:62-66 */
+enum { CIPHER_BC_NEUMANN = 0, CIPHER_BC_DIRICHLET = 1, CIPHER_BC_NONE = 2 };
+/* source_model_t, src/typedef.h:40-43 */
+enum { CIPHER_SOURCE_SINK = 0, CIPHER_SOURCE_RANDOM = 1 };
+/* arithmetic of the right-hand side */
+enum { CIPHER_FP64 = 0, CIPHER_FP32_MIXED = 1 };
+
+/* TSeries, src/typedef.h:145-150: sum_i coeff[i]*T^exp[i] + logcoeff*T*ln(T) */
+typedef struct cipher_tseries {
+    double coeff[CIPHER_MAXTSER];
+    double logcoeff;
+    int32_t exp[CIPHER_MAXTSER];
+    int32_t n;
+    int32_t pad_;
+} cipher_tseries;
+
+/* RK, src/typedef.h:153-156: n Redlich-Kister orders, enthalpy series
+ * tseries[ts .. ts+n-1], ternary component index rkindex[idx .. idx+n-1]. */
+typedef struct cipher_rk {
+    int32_t n;
+    int32_t ts;
+    int32_t idx;
+    int32_t pad_;
+} cipher_rk;
+
+/* TACTIVATIONPROP of a component mobility, src/typedef.h:159-163:
+ * unary: ncp series from tseries[unary]; binary: ncp*(ncp-1)/2 RK from rk[binary]. */
+typedef struct cipher_mobility {
+    double m0;
+    int32_t unary;
+    int32_t binary;
+} cipher_mobility;
+
+/* SOURCE, src/typedef.h:77-103 */
+typedef struct cipher_source {
+    double rate[CIPHER_MAXCP];
+    double ceq[CIPHER_MAXCP];
+    double amplitude[CIPHER_MAXCP];
+    int32_t model;
+    int32_t pad_;
+} cipher_source;
+
+/* MATERIAL, src/typedef.h:210-218 with the CHEMFE union of :166-201 flattened
+ * into table offsets.  Table order is the reference's loop order. */
+typedef struct cipher_material {
+    double molarvolume;
+    double chempot_ex_kineticcoeff;
+    double c0[2 * CIPHER_MAXCP];        /* calphad2sl: c0_p then c0_q */
+    double stoichiometry[2];            /* site multiplicities (p, q) */
+    double temperature0;
+    int32_t model;                      /* CIPHER_CHEM_* */
+    int32_t nsites;                     /* 0 none, 1 dis/quad, 2 calphad2sl */
+    int32_t ref;                        /* tseries: ref enthalpy (none/dis/quad) */
+    int32_t unary;                      /* tseries: ncp (dis/quad) or ncp*ncp (2sl, [cp*ncp+cq]) */
+    int32_t binary;                     /* rk: dis binary ncp(ncp-1)/2; 2sl binaryp ncp*ncp(ncp-1)/2 */
+    int32_t ternary;                    /* rk: dis ternary C(ncp,3); 2sl ternaryp C(ncp,3)*ncp */
+    int32_t binaryq;                    /* rk: 2sl binaryq ncp*ncp(ncp-1)/2 */
+    int32_t ternaryq;                   /* rk: 2sl ternaryq ncp*C(ncp,3) */
+    int32_t quad_binary;                /* tseries: quadratic binary, ncp */
+    int32_t quad_ceq;                   /* tseries: quadratic ceq, ncp */
+    int32_t mobility;                   /* first of ncp cipher_mobility */
+    int32_t specific_heat;              /* tseries (n==0: absent) */
+    int32_t latent_heat;                /* tseries (n==0: absent) */
+    int32_t tconductivity;              /* tseries (n==0: absent) */
+    int32_t nsources;
+    int32_t source;                     /* first cipher_source */
+} cipher_material;
+
+/* INTERFACE, src/typedef.h:221-240 */
+typedef struct cipher_interface {
+    double width;
+    double e0;                          /* ienergy->e->m0 */
+    double m0;                          /* imobility->m->m0 */
+    double potential[CIPHER_MAXCP];     /* segregation potential (ndp) */
+    double mobilityc[CIPHER_MAXCP];     /* interface solute mobility (ncp) */
+    int32_t e_act;                      /* tseries of the energy activation */
+    int32_t m_act;                      /* tseries of the mobility activation */
+    int32_t e_naniso, e_aniso;          /* anisotropy: aniso[e_aniso..] = dim*n dirs then n vals */
+    int32_t m_naniso, m_aniso;
+    int32_t has_potential;
+    int32_t has_mobilityc;
+    int32_t nsources;
+    int32_t source;
+} cipher_interface;
+
+/* BOUNDARYCONDITIONS, src/typedef.h:243-248 */
+typedef struct cipher_bc {
+    double chem_val[CIPHER_MAXCP];
+    double thermal_val;
+    int32_t boundary_id;                /* 1..2*dim, src/cipher.c:1214-1220 */
+    int32_t chem_type;                  /* CIPHER_BC_* */
+    int32_t thermal_type;
+    int32_t chem_mask;                  /* bit c set: component c constrained */
+} cipher_bc;
+
+/* The AppCtx fields the right-hand side reads (src/typedef.h:277-319). */
+typedef struct cipher_config {
+    const cipher_tseries *tseries;
+    const cipher_rk *rk;
+    const int32_t *rkindex;
+    const cipher_mobility *mobility;
+    const cipher_material *material;
+    const cipher_interface *interface;
+    const cipher_source *source;
+    const cipher_bc *bc;
+    const double *aniso;
+    const int32_t *phase_material;      /* npf */
+    const int32_t *interface_list;      /* npf*npf, upper triangle used */
+    double junctionpenalty;
+    double temperature_rate;            /* current load case */
+    int32_t has_temperature_rate;
+    int32_t dim, npf, ncp, nmat, nf, nbcs;
+    int32_t ntseries, nrk, nrkindex, nmobility, nsource, naniso;
+    int32_t max_phase_neighbours;       /* default CIPHER_MAXAP */
+    int32_t interpolation;              /* CIPHER_INTERP_* */
+    int32_t gradient_calculation;       /* must be 0 on the GPU path */
+} cipher_config;
+
+/* Field sizes/offsets of src/init.c:1330-1343. */
+typedef struct cipher_layout {
+    int32_t maxsites, sp_size, sf_size;
+    int32_t as_size, pf_size, dp_size, ex_size, tm_size;
+    int32_t as_offset, pf_offset, dp_offset, ex_offset, tm_offset;
+    int32_t ndof;                       /* dofs per cell */
+} cipher_layout;
+
+/* Flattened leaf mesh after an adapt (src/cipher.c:1187-1231).  Cells are the
+ * owned leaves; faces are the interior faces with two supports and no tree
+ * children, in the order the right-hand side visits them; (L,R) is the support
+ * order.  Boundary faces carry the label id of src/cipher.c:1214-1220. */
+typedef struct cipher_mesh {
+    const int32_t *face_cells;          /* 2*nfaces */
+    const double *cellgeom;             /* ncells: pow(volume, 1/dim) */
+    const int32_t *bface_cell;          /* nbfaces */
+    const int32_t *bface_id;            /* nbfaces */
+    int32_t ncells;
+    int32_t nfaces;
+    int32_t nbfaces;
+    int32_t pad_;
+} cipher_mesh;
+
+typedef struct cipher_ctx cipher_ctx;
+
+/* Layout of a config (no device work). */
+int cipher_layout_of(const cipher_config *cfg, cipher_layout *out);
+
+/* Create a context on the current HIP device: copies the config tables to HBM.
+ * precision is CIPHER_FP64 or CIPHER_FP32_MIXED. */
+int cipher_ctx_create(const cipher_config *cfg, int precision, cipher_ctx **out);
+int cipher_ctx_destroy(cipher_ctx *ctx);
+int cipher_ctx_layout(const cipher_ctx *ctx, cipher_layout *out);
+int cipher_set_precision(cipher_ctx *ctx, int precision);
+/* hipStream_t passed as void*; NULL selects the default stream. */
+int cipher_set_stream(cipher_ctx *ctx, void *stream);
+/* Update the load-case temperature rate (src/cipher.c:568). */
+int cipher_set_temperature_rate(cipher_ctx *ctx, int has_rate, double rate);
+
+/* Upload a flattened mesh; builds the per-cell face lists and preallocates
+ * every scratch buffer so that later calls allocate nothing. */
+int cipher_upload_mesh(cipher_ctx *ctx, const cipher_mesh *mesh);
+
+/* F = RHS(t, X): RHSFunction, src/cipher.c:27-681.  X_dev and F_dev hold
+ * ncells*ndof doubles; F is fully defined (AS and inactive slots are 0). */
+int cipher_rhs(cipher_ctx *ctx, double t, const double *X_dev, double *F_dev);
+
+/* Same as cipher_rhs and also writes, per cell, bit g of active_dev[cell]
+ * = active[g] of src/cipher.c:535-537 (0 for single-phase cells). */
+int cipher_rhs_active(cipher_ctx *ctx, double t, const double *X_dev,
+                      double *F_dev, uint32_t *active_dev);
+
+/* Active-set update of PostStep, src/cipher.c:869-974, in place on X_dev.
+ * Returns CIPHER_ERR_CAPACITY if a union exceeded PF_SIZE phases (the
+ * reference asserts, src/cipher.c:968); X_dev is then unspecified. */
+int cipher_poststep(cipher_ctx *ctx, double *X_dev);
+
+/* Initial state of SetUpProblem, src/init.c:1721-1830: phase_dev[cell] is the
+ * cell's phase label (0-based). */
+int cipher_setup_problem(cipher_ctx *ctx, const int32_t *phase_dev, double *X_dev);
+
+/* Weighted RMS error norm sqrt(1/N sum ((|u-y|)/(atol+rtol*max(|u|,|y|)))^2)
+ * over n doubles, written to *norm_out (host). */
+int cipher_wrms_norm(cipher_ctx *ctx, const double *U_dev, const double *Y_dev,
+                     int64_t n, double atol, double rtol, double *norm_out);
+
+/* Y = X + sum_j a[j]*K_j (j < nk), the RK stage / update combination, FP64.
+ * K_dev points to nk device vectors of n doubles each. */
+int cipher_axpy_n(cipher_ctx *ctx, int64_t n, double *Y_dev, const double *X_dev,
+                  int nk, const double *a, const double *const *K_dev);
+
+/* Message of the last failing call on this thread. */
+const char *cipher_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CIPHER_AMD_H */

@@ -1,0 +1,169 @@
+"""HIP path vs the CPU oracle on identical meshes and states (through the C ABI).
+
+Bars: FP64 residual within 1e-12 relative (per field, normwise: max|dF| over
+max|F|), active flags and active-set lists bit-exact, FP32-mixed residual
+within 1e-6 relative of FP64 with identical active flags.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from cipher_amd import abi
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+RTOL64 = 1e-12
+RTOL32 = 1e-6
+
+
+def _ctx(wl, precision=abi.FP64):
+    from cipher_amd.runtime import Context
+    ctx = Context(wl.cfg, precision=precision)
+    ctx.upload_mesh(wl.mesh)
+    return ctx
+
+
+def _dev(ctx, X):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(X)).to(ctx.device)
+
+
+def _host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+WORKLOADS = {
+    "gg2d": lambda: H.gg2d(),
+    "gg3d_amr": lambda: H.gg3d_amr(),
+    "ternary": lambda: H.ternary(),
+}
+
+
+def _states(wl):
+    X0 = H.initial_state(wl)
+    X1 = H.evolve(wl, X0, steps=6)
+    X2 = H.perturb(wl, X1)
+    return {"init": X0, "evolved": X1, "perturbed": X2}
+
+
+@pytest.mark.parametrize("name", list(WORKLOADS))
+def test_setup_problem_matches_oracle(name):
+    wl = WORKLOADS[name]()
+    print(f"[setup] {name} cells={wl.ncells}", flush=True)
+    ctx = _ctx(wl)
+    Xg = _host(ctx.setup_problem(wl.phase))
+    Xo = oracle.setup_problem(wl.cfg, wl.mesh, wl.phase)
+    m = H.state_mask(Xo, wl.layout, wl.cfg)
+    D = wl.layout["ndof"]
+    # active-set lists bit-exact
+    assert np.array_equal(Xg.reshape(-1, D)[:, :wl.layout["as_size"]][m.reshape(-1, D)[:, :wl.layout["as_size"]]],
+                          Xo.reshape(-1, D)[:, :wl.layout["as_size"]][m.reshape(-1, D)[:, :wl.layout["as_size"]]])
+    np.testing.assert_allclose(Xg[m], Xo[m], rtol=1e-13, atol=1e-300)
+
+
+@pytest.mark.parametrize("name", list(WORKLOADS))
+def test_rhs_fp64_matches_oracle(name):
+    import torch
+    wl = WORKLOADS[name]()
+    ctx = _ctx(wl)
+    for sname, X in _states(wl).items():
+        print(f"[rhs64] {name}/{sname} cells={wl.ncells}", flush=True)
+        Fo, ao = oracle.rhs(wl.cfg, wl.mesh, X, want_active=True)
+        act = torch.zeros(wl.ncells, dtype=torch.int32, device=ctx.device)
+        Fg = _host(ctx.rhs(_dev(ctx, X), active=act))
+        ag = _host(act).astype(np.uint32)
+        assert np.array_equal(ag, ao), f"{name}/{sname}: active flags differ in {np.count_nonzero(ag != ao)} cells"
+        errs = H.field_errors(Fg, Fo, wl.layout)
+        print("   errors", errs, flush=True)
+        for f, e in errs.items():
+            assert e <= RTOL64, f"{name}/{sname} field {f}: rel err {e:.3e}"
+        # AS entries and stale slots are exactly zero
+        D = wl.layout["ndof"]
+        assert np.all(Fg.reshape(-1, D)[:, :wl.layout["as_size"]] == 0.0)
+
+
+def test_rhs_grain_growth_bit_exact():
+    """No transcendental functions on the grain-growth path: identical bits."""
+    wl = H.gg2d()
+    ctx = _ctx(wl)
+    X = H.perturb(wl, H.evolve(wl, H.initial_state(wl), steps=4))
+    Fo = oracle.rhs(wl.cfg, wl.mesh, X)
+    Fg = _host(ctx.rhs(_dev(ctx, X)))
+    assert np.array_equal(Fg, Fo), f"max diff {np.abs(Fg - Fo).max():.3e}"
+
+
+@pytest.mark.parametrize("name", list(WORKLOADS))
+def test_poststep_matches_oracle(name):
+    wl = WORKLOADS[name]()
+    ctx = _ctx(wl)
+    L = wl.layout
+    D = L["ndof"]
+    X = H.evolve(wl, H.initial_state(wl), steps=3)
+    F = oracle.rhs(wl.cfg, wl.mesh, X)
+    Fv = F.reshape(-1, D)
+    m = np.abs(Fv[:, L["pf_offset"]:L["pf_offset"] + L["pf_size"]]).max()
+    Xs = X + (0.05 / m) * F           # a stage value with phases leaving the simplex
+    print(f"[post] {name} cells={wl.ncells}", flush=True)
+    Xo, rc = oracle.poststep(wl.cfg, wl.mesh, Xs)
+    assert rc == 0
+    Xg = _host(ctx.poststep(_dev(ctx, Xs)))
+    mo = H.state_mask(Xo, L, wl.cfg)
+    mg = H.state_mask(Xg, L, wl.cfg)
+    assert np.array_equal(mo, mg)
+    nas = L["as_size"]
+    assert np.array_equal(Xg.reshape(-1, D)[:, :nas], Xo.reshape(-1, D)[:, :nas] * mo.reshape(-1, D)[:, :nas]
+                          + Xg.reshape(-1, D)[:, :nas] * ~mo.reshape(-1, D)[:, :nas])
+    np.testing.assert_allclose(Xg[mo], Xo[mo], rtol=1e-13, atol=1e-300)
+
+
+def test_poststep_capacity_overflow_reports_error():
+    """More phases around a cell than the capacity: CIPHER_ERR_CAPACITY
+    (the reference asserts at src/cipher.c:968)."""
+    from cipher_amd import workload as W
+    txt = W.grain_growth_yaml(res=(16, 16), ngrains=24, capacity=2)
+    wl = W.Workload(txt, "cap")
+    ctx = _ctx(wl)
+    with pytest.raises(abi.CipherError) as e:
+        ctx.setup_problem(wl.phase)
+    assert e.value.code == abi.ERR_CAPACITY
+
+
+def test_wrms_norm_and_axpy():
+    import torch
+    rng = np.random.default_rng(3)
+    wl = H.gg2d()
+    ctx = _ctx(wl)
+    n = wl.ncells * wl.layout["ndof"]
+    U = rng.standard_normal(n)
+    Y = U + 1e-4 * rng.standard_normal(n)
+    v = ctx.wrms_norm(_dev(ctx, U), _dev(ctx, Y), 1e-4, 1e-4)
+    ref = oracle.wrms_norm(U, Y, 1e-4, 1e-4)
+    assert abs(v - ref) <= 1e-14 * ref
+    K = [rng.standard_normal(n) for _ in range(3)]
+    a = [0.5, -0.25, 0.125]
+    Yd = torch.empty(n, dtype=torch.float64, device=ctx.device)
+    ctx.axpy_n(Yd, _dev(ctx, U), a, [_dev(ctx, k) for k in K])
+    ref = U + ((a[0] * K[0] + a[1] * K[1]) + a[2] * K[2])
+    assert np.array_equal(_host(Yd), ref)
+
+
+@pytest.mark.parametrize("name", list(WORKLOADS))
+def test_rhs_fp32_mixed_within_1e6(name):
+    import torch
+    wl = WORKLOADS[name]()
+    c64 = _ctx(wl, abi.FP64)
+    c32 = _ctx(wl, abi.FP32_MIXED)
+    X = H.evolve(wl, H.initial_state(wl), steps=6)
+    Xd = _dev(c64, X)
+    a64 = torch.zeros(wl.ncells, dtype=torch.int32, device=c64.device)
+    a32 = torch.zeros_like(a64)
+    F64 = _host(c64.rhs(Xd, active=a64))
+    F32 = _host(c32.rhs(Xd, active=a32))
+    errs = H.field_errors(F32, F64, wl.layout)
+    print(f"[rhs32] {name} errors {errs}", flush=True)
+    for f, e in errs.items():
+        assert e <= RTOL32, f"{name} field {f}: FP32-mixed rel err {e:.3e}"
+    assert np.array_equal(_host(a32), _host(a64))
