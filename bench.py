@@ -1,0 +1,197 @@
+"""Benchmark: RHS cell-updates/s and HBM roofline of the phase-field RHS.
+
+One "step" is one evaluation of the right-hand side (cipher_rhs) over every
+leaf cell of the workload, inputs resident in HBM.  Default workload: 3D
+isotropic grain growth, 64^3-equivalent domain, AMR levels 3-6 refined at
+interfaces, 500 Voronoi grains, interface energies U[0.8,1.2] per pair
+(quantised to 255 types), capacity 8, FP64 drop-in; the FP32-mixed path is
+timed beside it on the same state and its error against FP64 is reported.
+
+The state is the SetUpProblem initial condition advanced by a few explicit
+steps on the GPU (diffuse interfaces).  The CPU baseline is the C oracle
+(single thread) on the same state, timed for a bounded number of calls.
+
+Prints one JSON line.  Run: python bench.py [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def build_workload(name):
+    from cipher_amd import workload as W
+    if name == "gg3d":
+        txt = W.grain_growth_yaml(res=(64, 64, 64), ngrains=500, periodic=False, energy_range=(0.8, 1.2),
+                                  ntypes=255, minlevel=3, t_end=50.0)
+        return W.Workload(txt, "grain_growth_3d_amr_64"), {
+            "workload": "grain_growth_3d_amr_64", "domain": "64^3", "amr_levels": "3-6",
+            "grains": 500, "capacity": 8, "components": 1}
+    if name == "gg2d":
+        txt = W.grain_growth_yaml(res=(128, 128), ngrains=20, periodic=True)
+        return W.Workload(txt, "grain_growth_2d_128"), {
+            "workload": "grain_growth_2d_128", "domain": "128^2 periodic", "grains": 20, "capacity": 8}
+    if name == "ternary":
+        txt = W.ternary_ppt_yaml(res=(128, 128, 128), ngrains=100, nppt=40, radius=3, minlevel=4)
+        return W.Workload(txt, "ternary_gb_precipitation_128"), {
+            "workload": "ternary_gb_precipitation_128", "domain": "128^3", "amr_levels": "4-7",
+            "grains": 100, "precipitates": 40, "components": 3, "capacity": 8, "T": 700.0}
+    raise SystemExit(f"unknown workload {name}")
+
+
+def algorithmic_bytes(ncells, nfaces, ndof):
+    """B_RHS = N_c*(16*D + 8) + 8*N_f: read X and write F once (8 B/dof each),
+    one FP64 cell size per cell, two int32 cell ids per interior face."""
+    return ncells * (16 * ndof + 8) + 8 * nfaces
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="gg3d")
+    ap.add_argument("--evolve", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the path shards by Morton ranges with a one-cell halo; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from cipher_amd import abi
+    from cipher_amd import workload as W
+    from cipher_amd.runtime import Context
+
+    t0 = time.time()
+    wl, cfgdesc = build_workload(args.workload)
+    L = wl.layout
+    D = L["ndof"]
+    P = L["pf_size"]
+    N, NF = wl.ncells, wl.mesh.nfaces
+    print(f"[bench] workload {wl.name}: {N} cells, {NF} faces, D={D}, built in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+
+    ctx = Context(wl.cfg, precision=abi.FP64)
+    ctx.upload_mesh(wl.mesh)
+    X = ctx.setup_problem(wl.phase)
+    if L["dp_size"]:
+        Xh = X.cpu().numpy()
+        W.uniform_mu_override(Xh, L, wl.phase, wl.cfg.phase_material)
+        X = torch.as_tensor(Xh).to(ctx.device)
+    F = torch.empty_like(X)
+    pf = slice(L["pf_offset"], L["pf_offset"] + P)
+    good = X.clone()
+    evolved = 0
+    for _ in range(args.evolve):
+        ctx.rhs(X, F)
+        m = F.view(-1, D)[:, pf].abs().max().item()
+        dt = 0.02 / m if m > 0 else 0.0
+        X.add_(F, alpha=dt)
+        try:
+            ctx.poststep(X)
+        except abi.CipherError as e:
+            if e.code != abi.ERR_CAPACITY:
+                raise
+            # the union outgrew the capacity (the reference asserts here):
+            # keep the last state whose active sets fit
+            X.copy_(good)
+            break
+        good.copy_(X)
+        evolved += 1
+    del good
+    torch.cuda.synchronize()
+    nact = np.bincount(np.rint(X.view(-1, D)[:, 0].cpu().numpy()).astype(int), minlength=P + 1)
+    print(f"[bench] state ready after {evolved} steps: active-set sizes {nact.tolist()}", file=sys.stderr, flush=True)
+
+    def time_mode(precision):
+        ctx.set_precision(precision)
+        for _ in range(args.warmup):
+            ctx.rhs(X, F)
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        stream = torch.cuda.current_stream()
+        ev0.record(stream)
+        t_wall = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.rhs(X, F)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        t_wall = time.perf_counter() - t_wall
+        ms = ev0.elapsed_time(ev1) / args.steps
+        return ms, t_wall, F.clone()
+
+    ms64, wall64, F64 = time_mode(abi.FP64)
+    print(f"[bench] fp64: {ms64:.4f} ms/RHS", file=sys.stderr, flush=True)
+    ms32, wall32, F32 = time_mode(abi.FP32_MIXED)
+    print(f"[bench] fp32-mixed: {ms32:.4f} ms/RHS", file=sys.stderr, flush=True)
+    ctx.set_precision(abi.FP64)
+
+    # FP32-mixed error against FP64, per field (normwise relative)
+    A, B = F32.view(-1, D).cpu().numpy(), F64.view(-1, D).cpu().numpy()
+    rel = {}
+    for name, sl in (("phase_fraction", pf), ("chemical_potential", slice(L["dp_offset"], L["dp_offset"] + L["dp_size"])),
+                     ("explicit_potential", slice(L["ex_offset"], L["ex_offset"] + L["ex_size"]))):
+        if A[:, sl].size:
+            s = np.abs(B[:, sl]).max()
+            rel[name] = float(np.abs(A[:, sl] - B[:, sl]).max() / s) if s > 0 else float(np.abs(A[:, sl]).max())
+
+    bytes_alg = algorithmic_bytes(N, NF, D)
+    rate64 = N / (ms64 * 1e-3)
+    rate32 = N / (ms32 * 1e-3)
+    gbs64 = bytes_alg / (ms64 * 1e-3) / 1e9
+    gbs32 = bytes_alg / (ms32 * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        Xh = X.cpu().numpy()
+        t_start = time.perf_counter()
+        calls = 0
+        while True:
+            oracle.rhs(wl.cfg, wl.mesh, Xh)
+            calls += 1
+            el = time.perf_counter() - t_start
+            if el >= args.cpu_seconds or calls >= 1000:
+                break
+        cpu = {"value": N * calls / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{calls} oracle RHS calls on the same {N}-cell state ({el:.1f} s, one thread)"}
+
+    out = {
+        "metric": "RHS cell-updates/s + %HBM peak: FP32-mixed vs FP64, <=1e-6 rel err",
+        "value": rate64,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms64,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Voronoi microstructure, reference YAML schema)",
+        "config": dict(cfgdesc, cells=N, faces=NF, dofs_per_cell=D, evolved_steps=evolved,
+                       active_set_sizes=nact.tolist(), parallelism="1 GPU"),
+        "roofline": {"bound": "hbm", "achieved": gbs64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs64 / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": bytes_alg},
+        "fp32_mixed": {"value": rate32, "ms_per_step": ms32, "achieved_gbs": gbs32,
+                       "frac": gbs32 / HBM_PEAK_GBS, "speedup_vs_fp64": ms64 / ms32, "rel_err_vs_fp64": rel},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -16,7 +16,16 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=o
          "-Wall", "-Wno-unused-variable", "-Wno-unused-function"]
 
 
-def build(verbose=False, jobs=4):
+def _stale(src, obj):
+    if not os.path.exists(obj):
+        return True
+    deps = [src] + [os.path.join(HERE, h) for h in os.listdir(HERE) if h.endswith(".h")]
+    deps.append(os.path.join(os.path.dirname(PKG), "include", "cipher_amd.h"))
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(verbose=False, jobs=4, force=False):
     objdir = os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
     procs = []
@@ -25,6 +34,8 @@ def build(verbose=False, jobs=4):
         src = os.path.join(HERE, s)
         obj = os.path.join(objdir, s + ".o")
         objs.append(obj)
+        if not force and not _stale(src, obj):
+            continue
         cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
@@ -55,4 +66,4 @@ def _wait(item):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, force="-f" in sys.argv))

@@ -89,6 +89,17 @@ __device__ inline R dtseries_deriv(const cipher_tseries *t, R T) {
     return r;
 }
 
+// active-set count/id read from a double in the state, clamped so that a
+// corrupt state can never index outside the tables or loop without bound
+__device__ __forceinline__ int clamp_count(double v, int P) {
+    const double r = round(v);
+    return r < 0.0 ? 0 : (r > (double)P ? P : (int)r);
+}
+__device__ __forceinline__ int clamp_id(double v, int npf) {
+    const double r = round(v);
+    return r < 0.0 ? 0 : (r > (double)(npf - 1) ? npf - 1 : (int)r);
+}
+
 template <int N> struct Cnt { static constexpr int v = N > 0 ? N : 1; };
 
 // Invertmatrix for n <= 4 (src/utility.c:232-405): adjugate of a matrix taken

@@ -22,28 +22,39 @@ __device__ __forceinline__ T psh(T v, int src) {
     return __shfl(v, src, G);
 }
 
-// Merge two ascending lists held one element per lane (A: na, B: nb) into
-// A.  Returns the union count (may exceed G: the caller flags overflow).
+// Merge two ascending duplicate-free lists held one element per lane (A: na,
+// B: nb) into A.  Ranks: a_s goes to s + #{non-duplicate b < a_s}; a
+// non-duplicate b_t goes to #{non-duplicate b before t} + #{a < b_t}.
+// Returns the union count (may exceed G: the caller flags overflow; lanes
+// at or beyond the count hold BIG).
 template <int G>
 __device__ __forceinline__ int group_union(int &a, int na, int b, int nb) {
     const int slot = threadIdx.x & (G - 1);
     const int BIG = 0x7fffffff;
     const int av = slot < na ? a : BIG;
     const int bv = slot < nb ? b : BIG;
-    int lessB = 0, lessA = 0, dupB = 0;
+    int lessA = 0, dupB = 0;
 #pragma unroll
     for (int t = 0; t < G; t++) {
-        const int bt = psh<G>(bv, t);
         const int at = psh<G>(av, t);
-        if (t < nb && bt < av) lessB++;
         if (t < na && at < bv) lessA++;
         if (t < na && at == bv) dupB = 1;
     }
-    const int rA = slot < na ? slot + lessB : -1;
-    const int rB = (slot < nb && !dupB) ? slot + lessA : -1;
-    int ndup = 0;
+    int lessB = 0, before = 0, ndup = 0;
 #pragma unroll
-    for (int t = 0; t < G; t++) ndup += (t < nb) ? psh<G>(dupB, t) : 0;
+    for (int t = 0; t < G; t++) {
+        const int bt = psh<G>(bv, t);
+        const int dt = psh<G>(dupB, t);
+        if (t < nb) {
+            ndup += dt;
+            if (!dt) {
+                if (bt < av) lessB++;
+                if (t < slot) before++;
+            }
+        }
+    }
+    const int rA = slot < na ? slot + lessB : -1;
+    const int rB = (slot < nb && !dupB) ? before + lessA : -1;
     const int cnt = na + nb - ndup;
     int out = BIG;
 #pragma unroll
@@ -63,9 +74,9 @@ __global__ __launch_bounds__(256) void k_project(double *X, int32_t *glist, long
     const int slot = threadIdx.x & (G - 1);
     if (cell >= ncells) return;
     double *x = X + cell * c.D;
-    const int n = (int)round(x[c.aso]);
+    const int n = clamp_count(x[c.aso], c.P);
     const bool on = slot < n;
-    const int myid = on ? (int)round(x[c.aso + 1 + slot]) : 0;
+    const int myid = on ? clamp_id(x[c.aso + 1 + slot], c.npf) : 0;
     int32_t *gl = glist + cell * (c.P + 1);
     if (n > 1) {
         const double v = on ? x[c.pfo + slot] : -INFINITY;
@@ -110,7 +121,7 @@ __global__ __launch_bounds__(256) void k_project(double *X, int32_t *glist, long
         if (slot == 0) {
             x[c.pfo] = 1.0;
             gl[0] = 1;
-            gl[1] = (int)round(x[c.aso + 1]);
+            gl[1] = clamp_id(x[c.aso + 1], c.npf);
         }
     }
 }
@@ -124,22 +135,22 @@ __global__ __launch_bounds__(256) void k_union(double *X, const int32_t *glist, 
     const int slot = threadIdx.x & (G - 1);
     if (cell >= ncells) return;
     double *x = X + cell * c.D;
-    const int n = (int)round(x[c.aso]);
+    const int n = clamp_count(x[c.aso], c.P);
     const bool on = slot < n;
-    const int oldid = on ? (int)round(x[c.aso + 1 + slot]) : -1;
+    const int oldid = on ? clamp_id(x[c.aso + 1 + slot], c.npf) : -1;
     const double oldphi = on ? x[c.pfo + slot] : 0.0;
     double oldex[SPV];
 #pragma unroll
     for (int k = 0; k < SPV; k++) oldex[k] = (on && k < c.SP) ? x[c.exo + slot * c.SP + k] : 0.0;
     const double T = x[c.tmo];
     const int32_t *gl = glist + cell * (c.P + 1);
-    int cnt = gl[0];
+    int cnt = min(max(gl[0], 0), c.P);
     int ss = slot < cnt ? gl[1 + slot] : 0;
     bool over = false;
     for (int k = cell_off[cell]; k < cell_off[cell + 1]; k++) {
         const long long nb = adj[k] >> 1;
         const int32_t *gn = glist + nb * (c.P + 1);
-        const int nn = gn[0];
+        const int nn = min(max(gn[0], 0), c.P);
         const int bv = slot < nn ? gn[1 + slot] : 0;
         cnt = group_union<G>(ss, cnt, bv, nn);
         if (cnt > c.P) { over = true; break; }
@@ -163,7 +174,7 @@ __global__ __launch_bounds__(256) void k_union(double *X, const int32_t *glist, 
         if (pos >= 0) {
             phiv = nphi;
         } else if constexpr (NC > 1) {
-            const cipher_material &m = c.mat[c.phase_material[ss]];
+            const cipher_material &m = c.mat[c.phase_material[min(max(ss, 0), c.npf - 1)]];
             double y[2 * NC];
 #pragma unroll
             for (int k = 0; k < 2 * NC; k++) y[k] = m.c0[k];
@@ -192,12 +203,12 @@ __global__ __launch_bounds__(256) void k_setup(double *X, const int32_t *phase, 
     const int slot = threadIdx.x & (G - 1);
     if (cell >= ncells) return;
     double *x = X + cell * c.D;
-    const int ph = phase[cell];
+    const int ph = min(max(phase[cell], 0), c.npf - 1);
     int cnt = 1;
     int ss = ph;
     bool over = false;
     for (int k = cell_off[cell]; k < cell_off[cell + 1]; k++) {
-        const int pn = phase[adj[k] >> 1];
+        const int pn = min(max(phase[adj[k] >> 1], 0), c.npf - 1);
         cnt = group_union<G>(ss, cnt, pn, 1);
         if (cnt > c.P) { over = true; break; }
     }
@@ -208,7 +219,7 @@ __global__ __launch_bounds__(256) void k_setup(double *X, const int32_t *phase, 
     const cipher_material &mown = c.mat[c.phase_material[ph]];
     const double T = mown.temperature0;
     if (slot < cnt) {
-        const cipher_material &m = c.mat[c.phase_material[ss]];
+        const cipher_material &m = c.mat[c.phase_material[min(max(ss, 0), c.npf - 1)]];
         x[c.aso + 1 + slot] = (double)ss;
         x[c.pfo + slot] = (ss == ph) ? 1.0 : 0.0;
         if constexpr (NC > 1) {

@@ -140,9 +140,9 @@ __global__ __launch_bounds__(256) void k_prep(const double *__restrict__ X, S *_
     if (lc >= nblk) return;
     const long long cell = c0 + lc;
     const double *x = lds + lc * c.D;
-    const int n = (int)round(x[c.aso]);
+    const int n = clamp_count(x[c.aso], c.P);
     const bool on = slot < n;
-    const int myid = on ? (int)round(x[c.aso + 1 + slot]) : 0;
+    const int myid = on ? clamp_id(x[c.aso + 1 + slot], c.npf) : 0;
     const R phi = on ? (R)x[c.pfo + slot] : R(0);
     const R T = (R)x[c.tmo];
     R hh, hsum, h;
@@ -246,10 +246,10 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<S> a, DevCfg c) {
         const double *x = ldsX + lc * c.D;
         double *fr = ldsF + lc * c.D;
         const int D = c.D;
-        const int n = (int)round(x[c.aso]);
+        const int n = clamp_count(x[c.aso], c.P);
         const bool on = slot < n;
-        const int myid = on ? (int)round(x[c.aso + 1 + slot]) : 0xFFFF;
-        const int id0 = (int)round(x[c.aso + 1]);
+        const int myid = on ? clamp_id(x[c.aso + 1 + slot], c.npf) : 0xFFFF;
+        const int id0 = clamp_id(x[c.aso + 1], c.npf);
         const double phid = on ? x[c.pfo + slot] : 0.0;
         const R phi = (R)phid;
         const R T = (R)x[c.tmo];
@@ -271,11 +271,11 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<S> a, DevCfg c) {
             const double ffd = 2.0 * (dL < dR ? vL / dL : vR / dR) / (dL + dR);
             const R ff = (R)ffd, volL = (R)vL, volR = (R)vR;
             const double *xn = a.X + nb * D;
-            const int nbn = (int)round(xn[c.aso]);
+            const int nbn = clamp_count(xn[c.aso], c.P);
             const int Ln = selfL ? n : nbn;
-            const int Lid0 = selfL ? id0 : (int)round(xn[c.aso + 1]);
+            const int Lid0 = selfL ? id0 : clamp_id(xn[c.aso + 1], c.npf);
             if (Ln > 1 || Lid0 > 1) {
-                const int nbid = slot < nbn ? (int)round(xn[c.aso + 1 + slot]) : 0xFFFE;
+                const int nbid = slot < nbn ? clamp_id(xn[c.aso + 1 + slot], c.npf) : 0xFFFE;
                 const R nbphi = slot < nbn ? (R)xn[c.pfo + slot] : R(0);
                 int pos = -1;
                 for (int t = 0; t < nbn; t++) {
