@@ -44,6 +44,8 @@ struct cipher_ctx {
     double *cellgeom = nullptr;
     void *cm = nullptr;
     int32_t *glist = nullptr;
+    int32_t *list = nullptr;        // FP32-mixed recomputation list (ncells)
+    int32_t *list_count = nullptr;
     int32_t *err = nullptr;
     double *partial = nullptr;
     double *norm = nullptr;
@@ -53,13 +55,14 @@ struct cipher_ctx {
 };
 
 static void free_mesh(cipher_ctx *c) {
-    void *ptrs[] = {c->cell_off, c->adj, c->bcell_off, c->bface_id, c->cellgeom, c->cm, c->glist};
+    void *ptrs[] = {c->cell_off, c->adj, c->bcell_off, c->bface_id, c->cellgeom, c->cm, c->glist, c->list};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->cell_off = c->adj = c->bcell_off = c->bface_id = nullptr;
     c->cellgeom = nullptr;
     c->cm = nullptr;
     c->glist = nullptr;
+    c->list = nullptr;
     c->ncells = 0;
 }
 
@@ -213,7 +216,7 @@ extern "C" int cipher_ctx_create(const cipher_config *cfg, int precision, cipher
         for (int s = 0; s < f.nsources; s++)
             if (cfg->source[f.source + s].model == CIPHER_SOURCE_SINK) d.has_sources = 1;
     }
-    if (hipMalloc(&c->err, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->partial, sizeof(double) * c->npart) != hipSuccess ||
+    if (hipMalloc(&c->err, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->list_count, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->partial, sizeof(double) * c->npart) != hipSuccess ||
         hipMalloc(&c->norm, sizeof(double)) != hipSuccess ||
         hipHostMalloc(&c->h_norm, sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&c->h_err, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
@@ -231,6 +234,7 @@ extern "C" int cipher_ctx_destroy(cipher_ctx *c) {
     free_mesh(c);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->err) (void)hipFree(c->err);
+    if (c->list_count) (void)hipFree(c->list_count);
     if (c->partial) (void)hipFree(c->partial);
     if (c->norm) (void)hipFree(c->norm);
     if (c->h_norm) (void)hipHostFree(c->h_norm);
@@ -298,7 +302,7 @@ extern "C" int cipher_upload_mesh(cipher_ctx *c, const cipher_mesh *m) {
     std::vector<int32_t> bpos(boff.begin(), boff.end() - 1);
     for (int k = 0; k < m->nbfaces; k++) bid[bpos[m->bface_cell[k]]++] = m->bface_id[k];
     free_mesh(c);
-    size_t cmrec = std::max(prep_record_bytes(c->dc.ncp, CIPHER_FP64), prep_record_bytes(c->dc.ncp, CIPHER_FP32_MIXED));
+    size_t cmrec = prep_record_bytes(c->dc.ncp);
     HIPCHK(hipMalloc(&c->cell_off, sizeof(int32_t) * (N + 1)));
     HIPCHK(hipMalloc(&c->adj, sizeof(int32_t) * adj.size()));
     HIPCHK(hipMalloc(&c->bcell_off, sizeof(int32_t) * (N + 1)));
@@ -306,6 +310,7 @@ extern "C" int cipher_upload_mesh(cipher_ctx *c, const cipher_mesh *m) {
     HIPCHK(hipMalloc(&c->cellgeom, sizeof(double) * std::max<long long>(N, 1)));
     HIPCHK(hipMalloc(&c->cm, cmrec * std::max<long long>(N, 1)));
     HIPCHK(hipMalloc(&c->glist, sizeof(int32_t) * (size_t)(c->L.pf_size + 1) * std::max<long long>(N, 1)));
+    HIPCHK(hipMalloc(&c->list, sizeof(int32_t) * std::max<long long>(N, 1)));
     HIPCHK(hipMemcpy(c->cell_off, off.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->adj, adj.data(), sizeof(int32_t) * adj.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->bcell_off, boff.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice));
@@ -329,6 +334,8 @@ static int rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active)
     L.bcell_off = c->bcell_off;
     L.bface_id = c->bface_id;
     L.cm = c->cm;
+    L.list = c->list;
+    L.list_count = c->list_count;
     L.ncells = c->ncells;
     cipher_status s = launch_rhs(L, c->precision, c->stream);
     if (s != CIPHER_OK) return fail(s, std::string("rhs launch: ") + hipGetErrorString(hipGetLastError()));

@@ -6,7 +6,6 @@
 
 namespace cipher {
 
-template <class S>
 struct RhsArgs {
     const double *X;
     double *F;
@@ -16,7 +15,9 @@ struct RhsArgs {
     const int32_t *adj;       // (neighbour << 1) | (self is R)
     const int32_t *bcell_off; // ncells+1: boundary faces of each cell
     const int32_t *bface_id;  // boundary label id per boundary-face entry
-    const S *cm;              // per-cell [comp(ncp), mob(ncp), tcond]
+    const double *cm;         // per-cell [comp(ncp), mob(ncp), tcond]
+    int32_t *list;            // FP32-mixed: cells recomputed in FP64
+    int32_t *list_count;
     long long ncells;
 };
 
@@ -28,11 +29,12 @@ struct RhsLaunch {
     const double *cellgeom;
     const int32_t *cell_off, *adj, *bcell_off, *bface_id;
     void *cm;
+    int32_t *list, *list_count;
     long long ncells;
 };
 
 cipher_status launch_rhs(const RhsLaunch &L, int precision, hipStream_t st);
-size_t prep_record_bytes(int ncp, int precision);
+size_t prep_record_bytes(int ncp);
 
 struct PostLaunch {
     const DevCfg *cfg;

@@ -3,7 +3,7 @@
 // Restates RHSFunction of micmog/CIPHER (src/cipher.c:27-681) as two kernels:
 //
 //   k_prep  per-cell composition, volume-ref mobility and thermal
-//           conductivity (src/cipher.c:149-233) into a compact scratch record,
+//           conductivity (src/cipher.c:149-233) into a compact FP64 record,
 //           only when the face fluxes need them (chemistry or conduction);
 //   k_rhs   per-cell gather over the cell's faces in face order (the
 //           reference scatters faces in the same order, so each cell sums its
@@ -15,7 +15,15 @@
 // reference's loop order through group shuffles, so FP64 results follow the
 // reference's rounding sequence.  A block's contiguous X records are staged in
 // LDS with coalesced loads and its F records are assembled in LDS and stored
-// coalesced, so the AoS state layout costs no extra HBM traffic.
+// coalesced.  Neighbour records are fetched a batch of faces at a time so the
+// dependent loads of several faces are in flight together.
+//
+// FP32-mixed: the face fluxes (Kahan-compensated), interpolants, capillary and
+// rate assembly run in FP32; the implicit site potentials, site fractions,
+// chemical energy, explicit potentials and their rate stay FP64.  Every slot
+// whose active flag is decided by the sign of an FP32 value lying inside its
+// rounding band is recomputed by the FP64 kernel (LIST mode), so the active
+// sets equal the FP64 path's.
 #include "kernels.h"
 
 namespace cipher {
@@ -44,7 +52,7 @@ __device__ __forceinline__ R itf_mobility(const DevCfg &c, int it, R T) {
     return m;
 }
 
-// Kahan-compensated accumulator (FP32 mode); plain sum in FP64 mode
+// Kahan-compensated accumulator in FP32; plain sum (reference order) in FP64
 template <class R>
 struct Acc {
     R s = R(0), comp = R(0);
@@ -60,25 +68,11 @@ struct Acc {
     }
 };
 
-// ---------------------------------------------------------------------------
-// block staging of contiguous AoS records
-__device__ __forceinline__ void stage_in(double *lds, const double *X, long long c0, int ncell_blk, int D) {
-    const long long base = c0 * D;
-    const int tot = ncell_blk * D;
-    for (int i = threadIdx.x; i < tot; i += blockDim.x) lds[i] = X[base + i];
-}
-__device__ __forceinline__ void stage_out(double *F, const double *lds, long long c0, int ncell_blk, int D) {
-    const long long base = c0 * D;
-    const int tot = ncell_blk * D;
-    for (int i = threadIdx.x; i < tot; i += blockDim.x) F[base + i] = lds[i];
-}
-
-// ---------------------------------------------------------------------------
 // per-cell interpolants of the active slots (src/utility.c:470-535)
 template <int G, class R>
-__device__ __forceinline__ void interpolant(const DevCfg &c, int n, R phi, R &hh, R &hsum, R &h) {
+__device__ __forceinline__ void interpolant(const DevCfg &c, int n, R phi, R &hh, R &h) {
     hh = (c.interp == CIPHER_INTERP_QUADRATIC) ? phi * phi : phi * phi * (R(3) - R(2) * phi);
-    hsum = R(0);
+    R hsum = R(0);
     for (int g = 0; g < n; g++) hsum += gsh<G>(hh, g);
     if (c.interp == CIPHER_INTERP_LINEAR) h = phi;
     else if (n == 1) h = R(1);
@@ -123,10 +117,44 @@ __device__ __forceinline__ void interface_mu(const DevCfg &c, int n, int myid, R
     }
 }
 
+// implicit site potentials and site fractions of the slot's phase, in FP64
+// (src/cipher.c:170-189 and 584-609): spim = nu_s (mu - mu_int) - EX
+template <int G, int NC>
+__device__ __forceinline__ void slot_sitefrac(const DevCfg &c, const cipher_material &m, bool on, int ns, int n,
+                                              int myid, double h, const double *x, int slot, double T, double *spim,
+                                              double *y) {
+    constexpr int ND = NC - 1;
+    double mui[Cnt<ND>::v];
+    interface_mu<G, NC, double>(c, n, myid, h, mui);
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int k = 0; k < ND; k++)
+            spim[s * ND + k] = (s < ns) ? m.stoichiometry[s] * (x[c.dpo + k] - mui[k]) - x[c.exo + slot * c.SP + s * ND + k]
+                                        : 0.0;
+#pragma unroll
+    for (int k = 0; k < 2 * NC; k++) y[k] = 0.0;
+    if (on) d_sitefrac<NC, double>(c, m, y, spim, T);
+}
+
+// ---------------------------------------------------------------------------
+// block staging of AoS records: contiguous (block of consecutive cells) or,
+// in LIST mode, each group copies its own cell's record
+__device__ __forceinline__ void stage_in(double *lds, const double *X, long long c0, int ncell_blk, int D) {
+    const long long base = c0 * D;
+    const int tot = ncell_blk * D;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) lds[i] = X[base + i];
+}
+__device__ __forceinline__ void stage_out(double *F, const double *lds, long long c0, int ncell_blk, int D) {
+    const long long base = c0 * D;
+    const int tot = ncell_blk * D;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) F[base + i] = lds[i];
+}
+
 // ---------------------------------------------------------------------------
 // k_prep: composition, mobility, conductivity per cell (src/cipher.c:149-233)
-template <int G, int NC, class R, class S>
-__global__ __launch_bounds__(256) void k_prep(const double *__restrict__ X, S *__restrict__ cm,
+template <int G, int NC>
+__global__ __launch_bounds__(256) void k_prep(const double *__restrict__ X, double *__restrict__ cm,
                                               long long ncells, DevCfg c) {
     constexpr int ND = NC - 1;
     constexpr int CPB = 256 / G;
@@ -143,50 +171,38 @@ __global__ __launch_bounds__(256) void k_prep(const double *__restrict__ X, S *_
     const int n = clamp_count(x[c.aso], c.P);
     const bool on = slot < n;
     const int myid = on ? clamp_id(x[c.aso + 1 + slot], c.npf) : 0;
-    const R phi = on ? (R)x[c.pfo + slot] : R(0);
-    const R T = (R)x[c.tmo];
-    R hh, hsum, h;
-    interpolant<G, R>(c, n, phi, hh, hsum, h);
-    if (!on) h = R(0);
+    const double phi = on ? x[c.pfo + slot] : 0.0;
+    const double T = x[c.tmo];
+    double hh, h;
+    interpolant<G, double>(c, n, phi, hh, h);
+    if (!on) h = 0.0;
     const int mat = c.phase_material[myid];
     const cipher_material &m = c.mat[mat];
-    R comp[NC], mob[NC];
-    R tcond = R(0);
+    double comp[NC], mob[NC];
+    double tcond = 0.0;
     if constexpr (NC > 1) {
-        R mui[Cnt<ND>::v];
-        interface_mu<G, NC, R>(c, n, myid, h, mui);
-        R spim[2 * Cnt<ND>::v], y[2 * NC], me[NC];
+        double spim[2 * Cnt<ND>::v], y[2 * NC], me[NC];
         const int ns = on ? m.nsites : 0;
-#pragma unroll
-        for (int s = 0; s < 2; s++)
-#pragma unroll
-            for (int k = 0; k < ND; k++) {
-                const R mu = (R)x[c.dpo + k];
-                spim[s * ND + k] = (s < ns) ? (R)m.stoichiometry[s] * (mu - mui[k]) - (R)x[c.exo + slot * c.SP + s * ND + k]
-                                            : R(0);
-            }
-#pragma unroll
-        for (int k = 0; k < 2 * NC; k++) y[k] = R(0);
+        slot_sitefrac<G, NC>(c, m, on, ns, n, myid, h, x, slot, T, spim, y);
         if (on) {
-            d_sitefrac<NC, R>(c, m, y, spim, T);
-            d_mobility<NC, R>(c, m, me, y, T);
+            d_mobility<NC, double>(c, m, me, y, T);
         } else {
 #pragma unroll
-            for (int k = 0; k < NC; k++) me[k] = R(0);
+            for (int k = 0; k < NC; k++) me[k] = 0.0;
         }
         // per-slot terms h*me and h*nu_s*y, summed in slot order
 #pragma unroll
-        for (int k = 0; k < NC; k++) { comp[k] = R(0); mob[k] = R(0); }
+        for (int k = 0; k < NC; k++) { comp[k] = 0.0; mob[k] = 0.0; }
         for (int g = 0; g < n; g++) {
             const int nsg = gsh<G>(ns, g);
 #pragma unroll
             for (int k = 0; k < NC; k++) mob[k] += gsh<G>(h * me[k], g);
 #pragma unroll
             for (int s = 0; s < 2; s++) {
-                const R st = on && s < ns ? (R)m.stoichiometry[s] : R(0);
+                const double st = on && s < ns ? m.stoichiometry[s] : 0.0;
 #pragma unroll
                 for (int k = 0; k < NC; k++) {
-                    const R v = gsh<G>(h * st * y[s * NC + k], g);
+                    const double v = gsh<G>(h * st * y[s * NC + k], g);
                     if (s < nsg) comp[k] += v;
                 }
             }
@@ -199,60 +215,136 @@ __global__ __launch_bounds__(256) void k_prep(const double *__restrict__ X, S *_
                     const cipher_interface &f = c.itf[c.ilist[idk * c.npf + idj]];
                     if (f.has_mobilityc) {
 #pragma unroll
-                        for (int k = 0; k < NC; k++) mob[k] *= dexp<R>((R)f.mobilityc[k] / (R)CRG / T);
+                        for (int k = 0; k < NC; k++) mob[k] *= exp(f.mobilityc[k] / CRG / T);
                     }
                 }
             }
         }
     }
     if (c.has_tcond) {
-        const cipher_material &mm = c.mat[mat];
-        R tv = (on && c.ts[mm.tconductivity].n) ? h * dtseries<R>(&c.ts[mm.tconductivity], T) : R(0);
+        const double tv = (on && c.ts[m.tconductivity].n) ? h * dtseries<double>(&c.ts[m.tconductivity], T) : 0.0;
         for (int g = 0; g < n; g++) {
             const int idg = gsh<G>(myid, g);
-            const R v = gsh<G>(tv, g);
+            const double v = gsh<G>(tv, g);
             if (c.ts[c.mat[c.phase_material[idg]].tconductivity].n) tcond += v;
         }
     }
     if (slot == 0) {
-        S *o = cm + cell * (2 * NC + 1);
+        double *o = cm + cell * (2 * NC + 1);
         if constexpr (NC > 1) {
 #pragma unroll
-            for (int k = 0; k < NC; k++) { o[k] = (S)comp[k]; o[NC + k] = (S)mob[k]; }
+            for (int k = 0; k < NC; k++) { o[k] = comp[k]; o[NC + k] = mob[k]; }
         }
-        o[2 * NC] = (S)tcond;
+        o[2 * NC] = tcond;
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_rhs: fused face gather + cell right-hand side
-template <int G, int NC, class R, class S>
-__global__ __launch_bounds__(256) void k_rhs(RhsArgs<S> a, DevCfg c) {
+// neighbour data of one face, fetched ahead of its use
+template <int NC>
+struct FaceIn {
+    int e;
+    double h, n, id0, id, ph, T;
+    double cm[NC > 1 ? 2 * NC + 1 : 1];
+    double mu[Cnt<NC - 1>::v];
+};
+
+template <int NC>
+__device__ __forceinline__ void load_face(FaceIn<NC> &f, const RhsArgs &a, const DevCfg &c, int e, int slot) {
+    constexpr int ND = NC - 1;
+    f.e = e;
+    const long long nb = e >> 1;
+    const double *xn = a.X + nb * c.D;
+    f.h = a.cellgeom[nb];
+    f.n = xn[c.aso];
+    f.id0 = xn[c.aso + 1];
+    f.id = slot < c.P ? xn[c.aso + 1 + slot] : 0.0;
+    f.ph = slot < c.P ? xn[c.pfo + slot] : 0.0;
+    if constexpr (NC > 1) {
+#pragma unroll
+        for (int q = 0; q < 2 * NC + 1; q++) f.cm[q] = a.cm[nb * (2 * NC + 1) + q];
+#pragma unroll
+        for (int q = 0; q < ND; q++) f.mu[q] = xn[c.dpo + q];
+    }
+    if (c.has_tcond) {
+        f.T = xn[c.tmo];
+        if constexpr (NC == 1) f.cm[0] = a.cm[nb * (2 * NC + 1) + 2 * NC];
+    }
+}
+
+// interface constants staged in LDS: [e0, width, m0] per interface type
+template <class R>
+__device__ __forceinline__ R pair_e(const DevCfg &c, const double *itab, int it, R T) {
+    if (c.has_iact) return itf_energy<R>(c, it, T);
+    R e = R(1);
+    e *= (R)itab[3 * it];
+    return e;
+}
+template <class R>
+__device__ __forceinline__ R pair_m(const DevCfg &c, const double *itab, int it, R T) {
+    if (c.has_iact) return itf_mobility<R>(c, it, T);
+    R m = R(1);
+    m *= (R)itab[3 * it + 2];
+    return m;
+}
+
+// make LDS writes of this wave visible to its other lanes
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// k_rhs: fused face gather + cell right-hand side.  LIST: process the cells
+// of a.list[0 .. *a.list_count) (the FP64 recomputation of FP32-mixed cells
+// whose active flags were inside the rounding band).
+template <int G, int NC, class R, bool LIST>
+__global__ __launch_bounds__(256) void k_rhs(RhsArgs a, DevCfg c) {
     constexpr int ND = NC - 1;
     constexpr int NDV = Cnt<ND>::v;
     constexpr int CPB = 256 / G;
+    constexpr bool MIXED = sizeof(R) == 4;
     extern __shared__ double lds[];
     double *ldsX = lds;
     double *ldsF = lds + CPB * c.D;
-    const long long c0 = (long long)blockIdx.x * CPB;
-    const int nblk = (int)min((long long)CPB, a.ncells - c0);
-    stage_in(ldsX, a.X, c0, nblk, c.D);
-    for (int i = threadIdx.x; i < nblk * c.D; i += blockDim.x) ldsF[i] = 0.0;
-    __syncthreads();
+    double *itab = ldsF + CPB * c.D;                       // 3*nf doubles
+    int *ptype = (int *)(itab + 3 * c.nf);                 // CPB*G*G ints
     const int lc = threadIdx.x / G;
     const int slot = threadIdx.x & (G - 1);
+    const int D = c.D;
+    for (int i = threadIdx.x; i < c.nf; i += blockDim.x) {
+        itab[3 * i] = c.itf[i].e0;
+        itab[3 * i + 1] = c.itf[i].width;
+        itab[3 * i + 2] = c.itf[i].m0;
+    }
+    int *pt = ptype + lc * G * G;
+    const long long limit = LIST ? (long long)*a.list_count : a.ncells;
+    const long long stride = (long long)gridDim.x * CPB;
+    for (long long c0 = (long long)blockIdx.x * CPB; c0 < limit; c0 += stride) {
+    const int nblk = (int)min((long long)CPB, limit - c0);
+    long long cell = -1;
+    if (lc < nblk) cell = LIST ? (long long)a.list[c0 + lc] : c0 + lc;
+    if (LIST) {
+        __syncthreads();
+        if (lc < nblk)
+            for (int k = slot; k < D; k += G) ldsX[lc * D + k] = a.X[cell * D + k];
+    } else {
+        stage_in(ldsX, a.X, c0, nblk, D);
+    }
+    for (int i = threadIdx.x; i < nblk * D; i += blockDim.x) ldsF[i] = 0.0;
+    __syncthreads();
     if (lc < nblk) {
-        const long long cell = c0 + lc;
-        const double *x = ldsX + lc * c.D;
-        double *fr = ldsF + lc * c.D;
-        const int D = c.D;
+        const double *x = ldsX + lc * D;
+        double *fr = ldsF + lc * D;
         const int n = clamp_count(x[c.aso], c.P);
         const bool on = slot < n;
         const int myid = on ? clamp_id(x[c.aso + 1 + slot], c.npf) : 0xFFFF;
         const int id0 = clamp_id(x[c.aso + 1], c.npf);
         const double phid = on ? x[c.pfo + slot] : 0.0;
         const R phi = (R)phid;
-        const R T = (R)x[c.tmo];
+        const double Td = x[c.tmo];
+        const R T = (R)Td;
         const double hS = a.cellgeom[cell];
         const int dim = c.dim;
 
@@ -260,82 +352,108 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<S> a, DevCfg c) {
         Acc<R> lap;
         Acc<R> dlap[NDV];
         Acc<R> tlap;
+        R lapabs = R(0);
+        double cmS[2 * NC + 1];
+        if (NC > 1 || c.has_tcond) {
+#pragma unroll
+            for (int q = 0; q < 2 * NC + 1; q++) cmS[q] = a.cm[cell * (2 * NC + 1) + q];
+        }
         const int kb = a.cell_off[cell], ke = a.cell_off[cell + 1];
+        // adjacency of the first 2G faces, one entry per lane
+        const int adj0 = (kb + slot < ke) ? a.adj[kb + slot] : 0;
+        const int adj1 = (kb + G + slot < ke) ? a.adj[kb + G + slot] : 0;
+        FaceIn<NC> cur, nxt;
+        if (kb < ke) load_face<NC>(cur, a, c, gsh<G>(adj0, 0), slot);
         for (int k = kb; k < ke; k++) {
-            const int e = a.adj[k];
-            const long long nb = e >> 1;
-            const bool selfL = (e & 1) == 0;
-            const double hN = a.cellgeom[nb];
-            const double dL = selfL ? hS : hN, dR = selfL ? hN : hS;
-            const double vL = dfastpow(dL, (unsigned)dim), vR = dfastpow(dR, (unsigned)dim);
-            const double ffd = 2.0 * (dL < dR ? vL / dL : vR / dR) / (dL + dR);
-            const R ff = (R)ffd, volL = (R)vL, volR = (R)vR;
-            const double *xn = a.X + nb * D;
-            const int nbn = clamp_count(xn[c.aso], c.P);
-            const int Ln = selfL ? n : nbn;
-            const int Lid0 = selfL ? id0 : clamp_id(xn[c.aso + 1], c.npf);
-            if (Ln > 1 || Lid0 > 1) {
-                const int nbid = slot < nbn ? clamp_id(xn[c.aso + 1 + slot], c.npf) : 0xFFFE;
-                const R nbphi = slot < nbn ? (R)xn[c.pfo + slot] : R(0);
-                int pos = -1;
-                for (int t = 0; t < nbn; t++) {
-                    const int v = gsh<G>(nbid, t);
-                    if (v == myid) pos = t;
+            if (k + 1 < ke) {
+                const int r = k + 1 - kb;
+                const int e1 = r < G ? gsh<G>(adj0, r) : (r < 2 * G ? gsh<G>(adj1, r - G) : a.adj[k + 1]);
+                load_face<NC>(nxt, a, c, e1, slot);
+            }
+            {
+                const bool selfL = (cur.e & 1) == 0;
+                const double hN = cur.h;
+                const double dL = selfL ? hS : hN, dR = selfL ? hN : hS;
+                const double vL = dfastpow(dL, (unsigned)dim), vR = dfastpow(dR, (unsigned)dim);
+                const double ffd = 2.0 * (dL < dR ? vL / dL : vR / dR) / (dL + dR);
+                const R ff = (R)ffd, volL = (R)vL, volR = (R)vR;
+                const int nbn = clamp_count(cur.n, c.P);
+                const int Ln = selfL ? n : nbn;
+                const int Lid0 = selfL ? id0 : clamp_id(cur.id0, c.npf);
+                if (Ln > 1 || Lid0 > 1) {
+                    const int nbid = slot < nbn ? clamp_id(cur.id, c.npf) : 0xFFFE;
+                    const R nbphi = slot < nbn ? (R)cur.ph : R(0);
+                    int pos = -1;
+                    for (int t = 0; t < nbn; t++) {
+                        const int v = gsh<G>(nbid, t);
+                        if (v == myid) pos = t;
+                    }
+                    const R pv = gsh<G>(nbphi, pos < 0 ? 0 : pos);
+                    if (on && pos >= 0) {
+                        R term;
+                        if (selfL) {
+                            const R fl = pv - phi;
+                            term = ff * fl / volL;
+                        } else {
+                            const R fl = phi - pv;
+                            term = -(ff * fl / volR);
+                        }
+                        lap.add(term);
+                        if constexpr (MIXED) lapabs += fabsf(term);
+                    }
                 }
-                const R pv = gsh<G>(nbphi, pos < 0 ? 0 : pos);
-                if (on && pos >= 0) {
-                    if (selfL) {
-                        const R fl = pv - phi;
-                        lap.add(ff * fl / volL);
-                    } else {
-                        const R fl = phi - pv;
-                        lap.add(-(ff * fl / volR));
+                if constexpr (NC > 1) {
+                    const double *cmL = selfL ? cmS : cur.cm, *cmR = selfL ? cur.cm : cmS;
+                    R me[NC], ca[NC], MB[ND * ND], dmu[ND], fd[ND];
+#pragma unroll
+                    for (int q = 0; q < NC; q++) {
+                        const double mLd = cmL[NC + q], mRd = cmR[NC + q];
+                        const R mL = (R)mLd, mR = (R)mRd;
+                        if (fabs(mLd) > 1e-32 && fabs(mRd) > 1e-32) {
+                            if constexpr (MIXED) me[q] = R(2) * mL * (mR / (mL + mR));   // no FP32 underflow
+                            else me[q] = R(2) * mL * mR / (mL + mR);
+                        } else {
+                            me[q] = R(0);
+                        }
+                        const R cL = (R)cmL[q], cR = (R)cmR[q];
+                        ca[q] = R(2) * cL * cR / (cL + cR);
+                    }
+#pragma unroll
+                    for (int i = 0; i < ND; i++)
+#pragma unroll
+                        for (int jj = 0; jj < ND; jj++) {
+                            R v = R(0);
+#pragma unroll
+                            for (int q = 0; q < NC; q++)
+                                v += ((jj == q ? R(1) : R(0)) - ca[jj]) * ((q == i ? R(1) : R(0)) - ca[i]) * me[q];
+                            MB[i * ND + jj] = v;
+                        }
+#pragma unroll
+                    for (int q = 0; q < ND; q++) {
+                        const double mus = x[c.dpo + q];
+                        dmu[q] = selfL ? (R)(cur.mu[q] - mus) : (R)(mus - cur.mu[q]);
+                    }
+                    dmatvec<ND, R>(fd, MB, dmu);
+#pragma unroll
+                    for (int q = 0; q < ND; q++) {
+                        if (selfL) dlap[q].add(ff * fd[q] / volL);
+                        else dlap[q].add(-(ff * fd[q] / volR));
+                    }
+                }
+                if (c.has_tcond) {
+                    const double tSd = cmS[2 * NC];
+                    const double tNd = cur.cm[NC > 1 ? 2 * NC : 0];
+                    const double tLd = selfL ? tSd : tNd, tRd = selfL ? tNd : tSd;
+                    if (fabs(tLd) > 1e-32 && fabs(tRd) > 1e-32) {
+                        const R tL = (R)tLd, tR = (R)tRd;
+                        const double TL = selfL ? Td : cur.T, TR = selfL ? cur.T : Td;
+                        const R ft = R(2) * tL * tR / (tL + tR) * (R)(TR - TL);
+                        if (selfL) tlap.add(ff * ft / volL);
+                        else tlap.add(-(ff * ft / volR));
                     }
                 }
             }
-            if constexpr (NC > 1) {
-                const S *cS = a.cm + cell * (2 * NC + 1), *cN = a.cm + nb * (2 * NC + 1);
-                const S *cmL = selfL ? cS : cN, *cmR = selfL ? cN : cS;
-                R me[NC], ca[NC], MB[ND * ND], dmu[ND], fd[ND];
-#pragma unroll
-                for (int q = 0; q < NC; q++) {
-                    const R mL = (R)cmL[NC + q], mR = (R)cmR[NC + q];
-                    if (fabs((double)mL) > 1e-32 && fabs((double)mR) > 1e-32) me[q] = R(2) * mL * mR / (mL + mR);
-                    else me[q] = R(0);
-                    const R cL = (R)cmL[q], cR = (R)cmR[q];
-                    ca[q] = R(2) * cL * cR / (cL + cR);
-                }
-#pragma unroll
-                for (int i = 0; i < ND; i++)
-#pragma unroll
-                    for (int j = 0; j < ND; j++) {
-                        R v = R(0);
-#pragma unroll
-                        for (int q = 0; q < NC; q++)
-                            v += ((j == q ? R(1) : R(0)) - ca[j]) * ((q == i ? R(1) : R(0)) - ca[i]) * me[q];
-                        MB[i * ND + j] = v;
-                    }
-                const double *xL = selfL ? x : xn, *xR = selfL ? xn : x;
-#pragma unroll
-                for (int q = 0; q < ND; q++) dmu[q] = (R)(xR[c.dpo + q] - xL[c.dpo + q]);
-                dmatvec<ND, R>(fd, MB, dmu);
-#pragma unroll
-                for (int q = 0; q < ND; q++) {
-                    if (selfL) dlap[q].add(ff * fd[q] / volL);
-                    else dlap[q].add(-(ff * fd[q] / volR));
-                }
-            }
-            if (c.has_tcond) {
-                const S *cS = a.cm + cell * (2 * NC + 1), *cN = a.cm + nb * (2 * NC + 1);
-                const R tS = (R)cS[2 * NC], tN = (R)cN[2 * NC];
-                const R tL = selfL ? tS : tN, tR = selfL ? tN : tS;
-                if (fabs((double)tL) > 1e-32 && fabs((double)tR) > 1e-32) {
-                    const double TL = selfL ? x[c.tmo] : xn[c.tmo], TR = selfL ? xn[c.tmo] : x[c.tmo];
-                    const R ft = R(2) * tL * tR / (tL + tR) * (R)(TR - TL);
-                    if (selfL) tlap.add(ff * ft / volL);
-                    else tlap.add(-(ff * ft / volR));
-                }
-            }
+            if (k + 1 < ke) cur = nxt;
         }
         // ---- boundary faces (src/cipher.c:320-389), per condition ----------
         if (c.nbcs > 0) {
@@ -353,11 +471,11 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<S> a, DevCfg c) {
                             fd[q] = R(0);
                             if (bc.chem_mask & (1 << q)) {
                                 if (bc.chem_type == CIPHER_BC_NEUMANN) fd[q] = (R)bc.chem_val[q] * dL;
-                                else fd[q] = (R)a.cm[cell * (2 * NC + 1) + NC + q] * ((R)bc.chem_val[q] - (R)x[c.dpo + q]);
+                                else fd[q] = (R)cmS[NC + q] * ((R)bc.chem_val[q] - (R)x[c.dpo + q]);
                             }
                         }
                     }
-                    const R tc = c.has_tcond ? (R)a.cm[cell * (2 * NC + 1) + 2 * NC] : R(0);
+                    const R tc = c.has_tcond ? (R)cmS[2 * NC] : R(0);
                     if (bc.thermal_type == CIPHER_BC_NEUMANN) ft = tc * (R)bc.thermal_val * dL;
                     else if (bc.thermal_type == CIPHER_BC_DIRICHLET) ft = tc * ((R)bc.thermal_val - T);
                     const R cf = R(1) / dfastpow(dL, 2u);
@@ -372,86 +490,100 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<S> a, DevCfg c) {
         const R plap = lap.s;
 
         // ---- cell (src/cipher.c:392-664) ------------------------------------
-        R hh, hsum, h;
-        interpolant<G, R>(c, n, phi, hh, hsum, h);
+        R hh, h;
+        interpolant<G, R>(c, n, phi, hh, h);
         if (!on) { h = R(0); hh = R(0); }
         const int mat = on ? c.phase_material[myid] : 0;
         const cipher_material &m = c.mat[mat];
         const int ns = on ? m.nsites : 0;
 
-        // site fractions of the slot's phase (same arithmetic as k_prep)
+        // site fractions of the slot's phase, FP64 (same arithmetic as k_prep)
+        double yd[2 * NC], spimd[2 * NDV];
+        slot_sitefrac<G, NC>(c, m, on, ns, n, myid, (double)h, x, slot, Td, spimd, yd);
         R y[2 * NC];
-        R mui[NDV];
-        R spim[2 * NDV];
 #pragma unroll
-        for (int k = 0; k < 2 * NC; k++) y[k] = R(0);
-        if constexpr (NC > 1) {
-            interface_mu<G, NC, R>(c, n, myid, h, mui);
+        for (int k = 0; k < 2 * NC; k++) y[k] = (R)yd[k];
+
+        // interface type of every active pair, row gk of the group's table
+        {
+            int row[G];
 #pragma unroll
-            for (int s = 0; s < 2; s++)
-#pragma unroll
-                for (int k = 0; k < ND; k++)
-                    spim[s * ND + k] = (s < ns) ? (R)m.stoichiometry[s] * ((R)x[c.dpo + k] - mui[k])
-                                                  - (R)x[c.exo + slot * c.SP + s * ND + k]
-                                                : R(0);
-            if (on) d_sitefrac<NC, R>(c, m, y, spim, T);
-        } else {
-            if (on) {
-                R dummy[2] = {R(0), R(0)};
-                d_sitefrac<NC, R>(c, m, y, dummy, T);
+            for (int gj = 0; gj < G; gj++) {
+                const int idj = gsh<G>(myid, gj);
+                row[gj] = (on && gj > slot && gj < n) ? c.ilist[myid * c.npf + idj] : 0;
             }
+#pragma unroll
+            for (int gj = 0; gj < G; gj++)
+                if (on && gj > slot && gj < n) pt[slot * G + gj] = row[gj];
         }
+        wave_sync();
 
         R pfrhs = R(0);
         bool act = false;
+        bool unsure = false;
         if (n > 1) {
             // capillary driving force (src/cipher.c:474-497), replayed per slot
-            R cap = R(0);
+            R cap = R(0), capabs = R(0);
             for (int gk = 0; gk < n; gk++) {
                 const R phk = gsh<G>(phi, gk), lapk = gsh<G>(plap, gk);
                 const int idk = gsh<G>(myid, gk);
                 for (int gj = gk + 1; gj < n; gj++) {
                     const R phj = gsh<G>(phi, gj), lapj = gsh<G>(plap, gj);
                     const int idj = gsh<G>(myid, gj);
-                    const int it = c.ilist[idk * c.npf + idj];
-                    const R e = itf_energy<R>(c, it, T);
-                    const R w = (R)c.itf[it].width;
-                    if (slot == gk) cap -= e * (phj / w + lapj * w / (R)CPI / (R)CPI);
-                    if (slot == gj) cap -= e * (phk / w + lapk * w / (R)CPI / (R)CPI);
+                    const int it = pt[gk * G + gj];
+                    const R e = pair_e<R>(c, itab, it, T);
+                    const R w = (R)itab[3 * it + 1];
+                    R lapaj = R(0), lapak = R(0);
+                    if constexpr (MIXED) { lapaj = gsh<G>(lapabs, gj); lapak = gsh<G>(lapabs, gk); }
+                    if (slot == gk) {
+                        cap -= e * (phj / w + lapj * w / (R)CPI / (R)CPI);
+                        if constexpr (MIXED) capabs += fabsf(e) * (fabsf(phj / w) + lapaj * fabsf(w) / (R)CPI / (R)CPI);
+                    }
+                    if (slot == gj) {
+                        cap -= e * (phk / w + lapk * w / (R)CPI / (R)CPI);
+                        if constexpr (MIXED) capabs += fabsf(e) * (fabsf(phk / w) + lapak * fabsf(w) / (R)CPI / (R)CPI);
+                    }
                     for (int gi = gj + 1; gi < n; gi++) {
                         const R phi_i = gsh<G>(phi, gi);
                         const int idi = gsh<G>(myid, gi);
-                        const int it2 = c.ilist[idk * c.npf + idi];
-                        const int it3 = c.ilist[idj * c.npf + idi];
+                        const int it2 = pt[gk * G + gi];
+                        const int it3 = pt[gj * G + gi];
                         R tj = R(0);
                         const R r1 = e / w;
                         tj = tj > r1 ? tj : r1;
-                        const R r2 = itf_energy<R>(c, it2, T) / (R)c.itf[it2].width;
+                        const R r2 = pair_e<R>(c, itab, it2, T) / (R)itab[3 * it2 + 1];
                         tj = tj > r2 ? tj : r2;
-                        const R r3 = itf_energy<R>(c, it3, T) / (R)c.itf[it3].width;
+                        const R r3 = pair_e<R>(c, itab, it3, T) / (R)itab[3 * it3 + 1];
                         tj = tj > r3 ? tj : r3;
                         tj *= (R)c.junctionpenalty;
-                        if (slot == gk) cap -= tj * phj * phi_i;
-                        if (slot == gj) cap -= tj * phk * phi_i;
-                        if (slot == gi) cap -= tj * phk * phj;
+                        R t3 = R(0);
+                        if (slot == gk) t3 = tj * phj * phi_i;
+                        if (slot == gj) t3 = tj * phk * phi_i;
+                        if (slot == gi) t3 = tj * phk * phj;
+                        if (slot == gk || slot == gj || slot == gi) {
+                            cap -= t3;
+                            if constexpr (MIXED) capabs += fabsf(t3);
+                        }
                     }
                 }
                 if (slot == gk) {
                     cap *= R(8);
                     cap += lapk;
+                    if constexpr (MIXED) capabs = R(8) * capabs + lapabs;
                 }
             }
-            // chemical driving force (src/cipher.c:499-522; Chemenergy, src/material.c:480-490)
+            // chemical driving force (src/cipher.c:499-522; Chemenergy, src/material.c:480-490), FP64
             R chs = R(0);
             if (on) {
-                chs = d_chemenergy<NC, R>(c, m, y, T);
-                chs /= (R)m.molarvolume;
+                double ch = d_chemenergy<NC, double>(c, m, yd, Td);
+                ch /= m.molarvolume;
 #pragma unroll
                 for (int s = 0; s < 2; s++) {
                     if (s >= ns) break;
 #pragma unroll
-                    for (int k = 0; k < ND; k++) chs -= (R)x[c.dpo + k] * (R)m.stoichiometry[s] * y[s * NC + k];
+                    for (int k = 0; k < ND; k++) ch -= x[c.dpo + k] * m.stoichiometry[s] * yd[s * NC + k];
                 }
+                chs = (R)ch;
             }
             if constexpr (NC > 1) {
                 if (c.has_potential) {
@@ -488,24 +620,43 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<S> a, DevCfg c) {
                     }
                 }
             }
+            R chsabs = R(0);
+            if constexpr (MIXED) {
+                // magnitude of the inputs of the interpolant-derivative product
+                R mx = R(0);
+                for (int g = 0; g < n; g++) mx = fmaxf(mx, fabsf(gsh<G>(chs, g)));
+                chsabs = R(12) * mx;
+            }
             chs = interp_deriv<G, R>(c, n, phi, hh, chs);
             // unconstrained rate and active flags (src/cipher.c:524-539)
-            R ru = R(0);
+            R ru = R(0), ruabs = R(0);
             R nact = R(0);
             for (int gk = 0; gk < n; gk++) {
                 const R capk = gsh<G>(cap, gk), chk = gsh<G>(chs, gk), hk = gsh<G>(h, gk);
                 const int idk = gsh<G>(myid, gk);
+                R capak = R(0);
+                if constexpr (MIXED) capak = gsh<G>(capabs, gk);
                 for (int gj = gk + 1; gj < n; gj++) {
                     const R capj = gsh<G>(cap, gj), chj = gsh<G>(chs, gj), hj = gsh<G>(h, gj);
                     const int idj = gsh<G>(myid, gj);
-                    const R mkj = itf_mobility<R>(c, c.ilist[idk * c.npf + idj], T);
+                    const R mkj = pair_m<R>(c, itab, pt[gk * G + gj], T);
                     const R rv = mkj * ((capk - capj) - (chk - chj) * R(8) * sqrt(hk * hj) / (R)CPI);
                     if (slot == gk) ru += rv;
                     if (slot == gj) ru -= rv;
+                    if constexpr (MIXED) {
+                        const R capaj = gsh<G>(capabs, gj);
+                        if (slot == gk || slot == gj)
+                            ruabs += fabsf(mkj) * (fabsf(capk) + fabsf(capj) + capak + capaj
+                                                   + (fabsf(chk) + fabsf(chj) + R(2) * chsabs) * R(8) * sqrtf(hk * hj) / (R)CPI);
+                    }
                 }
-                if (slot == gk)
+                if (slot == gk) {
                     act = (phid > CIPHER_TOL && phid < 1.0 - CIPHER_TOL) || (phid < CIPHER_TOL && ru > R(0))
                           || (phid > 1.0 - CIPHER_TOL && ru < R(0));
+                    // the decision reads the sign of ru: is it inside the FP32 band?
+                    if constexpr (MIXED)
+                        unsure = !(phid > CIPHER_TOL && phid < 1.0 - CIPHER_TOL) && fabsf(ru) <= 1.0e-4f * ruabs;
+                }
             }
             for (int g = 0; g < n; g++) nact += gsh<G>((int)act, g) ? R(1) : R(0);
             // constrained rate (src/cipher.c:541-555)
@@ -519,7 +670,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<S> a, DevCfg c) {
                     const R capj = gsh<G>(cap, gj), chj = gsh<G>(chs, gj), hj = gsh<G>(h, gj);
                     const int idj = gsh<G>(myid, gj);
                     if (!aj) continue;
-                    const R mkj = itf_mobility<R>(c, c.ilist[idk * c.npf + idj], T);
+                    const R mkj = pair_m<R>(c, itab, pt[gk * G + gj], T);
                     const R rv = mkj * ((capk - capj) - (chk - chj) * R(8) * sqrt(hk * hj) / (R)CPI);
                     if (slot == gk) pfrhs += rv;
                     if (slot == gj) pfrhs -= rv;
@@ -527,10 +678,20 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<S> a, DevCfg c) {
                 if (slot == gk) pfrhs /= nact;
             }
         }
+        const int gbase = (threadIdx.x & 63) & ~(G - 1);
+        const unsigned long long gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << gbase);
+        if constexpr (MIXED && !LIST) {
+            if (__ballot(unsure) & gmask) {
+                // leave this cell to the FP64 pass
+                if (slot == 0) {
+                    const int k = atomicAdd(a.list_count, 1);
+                    a.list[k] = (int32_t)cell;
+                }
+            }
+        }
         if (a.active) {
             const unsigned long long bal = __ballot(act);
-            const int gbase = (threadIdx.x & 63) & ~(G - 1);
-            if (slot == 0) a.active[cell] = (uint32_t)((bal >> gbase) & ((G == 64) ? ~0ull : ((1ull << G) - 1)));
+            if (slot == 0) a.active[cell] = (uint32_t)((bal & gmask) >> gbase);
         }
 
         // temperature rate (src/cipher.c:558-568)
@@ -550,148 +711,150 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<S> a, DevCfg c) {
             if (c.has_trate) tmrhs += (R)c.trate;
         }
 
-        R dprhs[NDV];
+        double dprhs[NDV];
 #pragma unroll
-        for (int k = 0; k < NDV; k++) dprhs[k] = R(0);
-        R cprhs[2 * NDV];
+        for (int k = 0; k < NDV; k++) dprhs[k] = 0.0;
+        double cprd[2 * NDV];
 #pragma unroll
-        for (int k = 0; k < 2 * NDV; k++) cprhs[k] = R(0);
+        for (int k = 0; k < 2 * NDV; k++) cprd[k] = 0.0;
         if constexpr (NC > 1) {
-            // explicit potential rate (src/cipher.c:572-581)
-            R spw[2 * NDV];
+            // explicit potential rate (src/cipher.c:572-581), FP64
             if (on) {
-                d_sitepot_explicit<NC, R>(c, m, spw, y, T);
+                double spw[2 * NDV];
+                d_sitepot_explicit<NC, double>(c, m, spw, yd, Td);
 #pragma unroll
                 for (int s = 0; s < 2; s++) {
                     if (s >= ns) break;
 #pragma unroll
                     for (int k = 0; k < ND; k++)
-                        cprhs[s * ND + k] = (R)m.chempot_ex_kineticcoeff
-                                            * (spw[s * ND + k] - (R)x[c.exo + slot * c.SP + s * ND + k]);
+                        cprd[s * ND + k] = m.chempot_ex_kineticcoeff * (spw[s * ND + k] - x[c.exo + slot * c.SP + s * ND + k]);
                 }
             }
-            // composition rate (src/cipher.c:584-662)
-            R wcp[NDV];
+            // composition rate (src/cipher.c:584-662), FP64 in both modes
+            const double hq = (double)h, hhq = (double)hh, pfq = (double)pfrhs, tmq = (double)tmrhs;
+            double wcp[NDV];
 #pragma unroll
-            for (int k = 0; k < ND; k++) wcp[k] = R(0);
+            for (int k = 0; k < ND; k++) wcp[k] = 0.0;
             if (c.has_potential) {
                 for (int gk = 0; gk < n; gk++) {
-                    const R hk = gsh<G>(h, gk);
+                    const double hk = gsh<G>(hq, gk);
                     const int idk = gsh<G>(myid, gk);
                     for (int gj = gk + 1; gj < n; gj++) {
-                        const R hj = gsh<G>(h, gj);
+                        const double hj = gsh<G>(hq, gj);
                         const int idj = gsh<G>(myid, gj);
                         const cipher_interface &f = c.itf[c.ilist[idk * c.npf + idj]];
                         if (f.has_potential) {
 #pragma unroll
                             for (int k = 0; k < ND; k++) {
-                                if (slot == gk) wcp[k] += (R)f.potential[k] * hj;
-                                if (slot == gj) wcp[k] += (R)f.potential[k] * hk;
+                                if (slot == gk) wcp[k] += f.potential[k] * hj;
+                                if (slot == gj) wcp[k] += f.potential[k] * hk;
                             }
                         }
                     }
                 }
             }
-            R tpot[2 * ND * ND], tT[2 * ND];
-            R exs = R(0);
+            double tpot[2 * ND * ND], tT[2 * ND];
+            double exs = 0.0;
 #pragma unroll
-            for (int k = 0; k < 2 * ND * ND; k++) tpot[k] = R(0);
+            for (int k = 0; k < 2 * ND * ND; k++) tpot[k] = 0.0;
 #pragma unroll
-            for (int k = 0; k < 2 * ND; k++) tT[k] = R(0);
-            if (on) d_sitefrac_tangent<NC, R>(c, m, tpot, tT, exs, y, spim, T);
-            R dcdm[ND * ND];
+            for (int k = 0; k < 2 * ND; k++) tT[k] = 0.0;
+            if (on) {
+                d_sitefrac_tangent<NC, double>(c, m, tpot, tT, exs, yd, spimd, Td);
+            }
+            double dcdm[ND * ND];
 #pragma unroll
-            for (int q = 0; q < ND * ND; q++) dcdm[q] = R(0);
+            for (int q = 0; q < ND * ND; q++) dcdm[q] = 0.0;
             for (int g = 0; g < n; g++) {
                 const int nsg = gsh<G>(ns, g);
 #pragma unroll
                 for (int s = 0; s < 2; s++) {
-                    const R st = (s < ns) ? (R)m.stoichiometry[s] : R(0);
+                    const double st = (s < ns) ? m.stoichiometry[s] : 0.0;
 #pragma unroll
                     for (int q = 0; q < ND * ND; q++) {
-                        const R v = gsh<G>(h * st * st * tpot[s * ND * ND + q], g);
+                        const double v = gsh<G>(hq * st * st * tpot[s * ND * ND + q], g);
                         if (s < nsg) dcdm[q] += v;
                     }
                 }
             }
             {
-                R tmp[ND];
+                double tmp[ND];
 #pragma unroll
                 for (int k = 0; k < ND; k++) tmp[k] = wcp[k];
-                dmatvec<ND, R>(wcp, dcdm, tmp);
+                dmatvec<ND, double>(wcp, dcdm, tmp);
             }
-            R wdp[ND];
+            double wdp[ND];
 #pragma unroll
-            for (int k = 0; k < ND; k++) wdp[k] = R(0);
+            for (int k = 0; k < ND; k++) wdp[k] = 0.0;
             if (n > 1) {
 #pragma unroll
                 for (int q = 0; q < ND; q++) {
-                    R wpf = R(0);
+                    double wpf = 0.0;
 #pragma unroll
                     for (int s = 0; s < 2; s++)
-                        if (s < ns) wpf += (R)m.stoichiometry[s] * y[s * NC + q];
+                        if (s < ns) wpf += m.stoichiometry[s] * yd[s * NC + q];
                     wpf -= wcp[q];
-                    if (!on) wpf = R(0);
-                    wpf = interp_deriv<G, R>(c, n, phi, hh, wpf);
-                    for (int g = 0; g < n; g++) wdp[q] -= gsh<G>(pfrhs * wpf, g);
+                    if (!on) wpf = 0.0;
+                    wpf = interp_deriv<G, double>(c, n, phid, hhq, wpf);
+                    for (int g = 0; g < n; g++) wdp[q] -= gsh<G>(pfq * wpf, g);
                 }
             }
             // Chemsource (src/material.c:321-351), called with the interpolants
-            R csrc[ND];
+            double csrc[ND];
 #pragma unroll
-            for (int k = 0; k < ND; k++) csrc[k] = R(0);
+            for (int k = 0; k < ND; k++) csrc[k] = 0.0;
             if (c.has_sources) {
-                const S *cmc = a.cm + cell * (2 * NC + 1);
                 for (int gk = 0; gk < n; gk++) {
                     const int idk = gsh<G>(myid, gk);
-                    const R hk = gsh<G>(h, gk);
+                    const double hk = gsh<G>(hq, gk);
                     const cipher_material &mk = c.mat[c.phase_material[idk]];
                     for (int s = 0; s < mk.nsources; s++) {
                         const cipher_source &src = c.src[mk.source + s];
                         if (src.model != CIPHER_SOURCE_SINK) continue;
 #pragma unroll
                         for (int q = 0; q < ND; q++)
-                            csrc[q] += hk * hk * ((R)src.rate[q] * ((R)src.ceq[q] - (R)cmc[q]));
+                            csrc[q] += hk * hk * (src.rate[q] * (src.ceq[q] - cmS[q]));
                     }
                     for (int gj = gk + 1; gj < n; gj++) {
                         const int idj = gsh<G>(myid, gj);
-                        const R hj = gsh<G>(h, gj);
+                        const double hj = gsh<G>(hq, gj);
                         const cipher_interface &f = c.itf[c.ilist[idk * c.npf + idj]];
                         for (int s = 0; s < f.nsources; s++) {
                             const cipher_source &src = c.src[f.source + s];
                             if (src.model != CIPHER_SOURCE_SINK) continue;
 #pragma unroll
                             for (int q = 0; q < ND; q++)
-                                csrc[q] += R(2) * hk * hj * ((R)src.rate[q] * ((R)src.ceq[q] - (R)cmc[q]));
+                                csrc[q] += 2.0 * hk * hj * (src.rate[q] * (src.ceq[q] - cmS[q]));
                         }
                     }
                 }
             }
 #pragma unroll
-            for (int q = 0; q < ND; q++) wdp[q] += (dlap[q].s + csrc[q]);
+            for (int q = 0; q < ND; q++) wdp[q] += ((double)dlap[q].s + csrc[q]);
             // sum_g sum_s h nu (T_ex . EXdot + T_T * Tdot)
             for (int g = 0; g < n; g++) {
                 const int nsg = gsh<G>(ns, g);
 #pragma unroll
                 for (int s = 0; s < 2; s++) {
-                    R tv[ND];
-                    {
-                        R tex[ND * ND];
+                    double tv[ND];
+                    double tex[ND * ND];
 #pragma unroll
-                        for (int q = 0; q < ND * ND; q++) tex[q] = exs == R(0) ? R(0) : -tpot[s * ND * ND + q];
-                        dmatvec<ND, R>(tv, tex, &cprhs[s * ND]);
-                    }
-                    const R st = (s < ns) ? (R)m.stoichiometry[s] : R(0);
+                    for (int q = 0; q < ND * ND; q++) tex[q] = exs == 0.0 ? 0.0 : -tpot[s * ND * ND + q];
+                    double cps[ND];
+#pragma unroll
+                    for (int q = 0; q < ND; q++) cps[q] = cprd[s * ND + q];
+                    dmatvec<ND, double>(tv, tex, cps);
+                    const double st = (s < ns) ? m.stoichiometry[s] : 0.0;
 #pragma unroll
                     for (int q = 0; q < ND; q++) {
-                        const R v = gsh<G>(h * st * (tv[q] + tT[s * ND + q] * tmrhs), g);
+                        const double v = gsh<G>(hq * st * (tv[q] + tT[s * ND + q] * tmq), g);
                         if (s < nsg) wdp[q] -= v;
                     }
                 }
             }
-            R dmdc[ND * ND];
-            dinvert<ND, R>(dmdc, dcdm);
-            dmatvec<ND, R>(dprhs, dmdc, wdp);
+            double dmdc[ND * ND];
+            dinvert<ND, double>(dmdc, dcdm);
+            dmatvec<ND, double>(dprhs, dmdc, wdp);
         }
 
         // ---- assemble F in LDS ------------------------------------------------
@@ -699,7 +862,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<S> a, DevCfg c) {
         if (slot == 0) {
             fr[c.tmo] = (double)tmrhs;
 #pragma unroll
-            for (int q = 0; q < ND; q++) fr[c.dpo + q] = (double)dprhs[q];
+            for (int q = 0; q < ND; q++) fr[c.dpo + q] = dprhs[q];
         }
         if constexpr (NC > 1) {
             if (on) {
@@ -707,18 +870,26 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<S> a, DevCfg c) {
                 for (int s = 0; s < 2; s++) {
                     if (s >= ns) break;
 #pragma unroll
-                    for (int q = 0; q < ND; q++) fr[c.exo + slot * c.SP + s * ND + q] = (double)cprhs[s * ND + q];
+                    for (int q = 0; q < ND; q++) fr[c.exo + slot * c.SP + s * ND + q] = cprd[s * ND + q];
                 }
             }
         }
     }
     __syncthreads();
-    stage_out(a.F, ldsF, c0, nblk, c.D);
+    if (LIST) {
+        if (lc < nblk)
+            for (int k = slot; k < D; k += G) a.F[cell * D + k] = ldsF[lc * D + k];
+    } else {
+        stage_out(a.F, ldsF, c0, nblk, D);
+    }
+    }  // block loop
 }
 
+__global__ void k_zero_i32(int32_t *p) { *p = 0; }
+
 // ---------------------------------------------------------------------------
-template <int G, int NC, class R, class S>
-static cipher_status launch_G(const RhsLaunch &L, hipStream_t st) {
+template <int G, int NC>
+static cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st) {
     constexpr int CPB = 256 / G;
     const long long nb = (L.ncells + CPB - 1) / CPB;
     if (nb == 0) return CIPHER_OK;
@@ -726,45 +897,47 @@ static cipher_status launch_G(const RhsLaunch &L, hipStream_t st) {
     const bool need_prep = (NC > 1) || c.has_tcond;
     if (need_prep) {
         const size_t sh = (size_t)CPB * c.D * sizeof(double);
-        hipLaunchKernelGGL((k_prep<G, NC, R, S>), dim3((unsigned)nb), dim3(256), sh, st, L.X, (S *)L.cm, L.ncells, c);
+        hipLaunchKernelGGL((k_prep<G, NC>), dim3((unsigned)nb), dim3(256), sh, st, L.X, (double *)L.cm, L.ncells, c);
     }
-    RhsArgs<S> a;
+    RhsArgs a;
     a.X = L.X; a.F = L.F; a.active = L.active; a.cellgeom = L.cellgeom;
     a.cell_off = L.cell_off; a.adj = L.adj; a.bcell_off = L.bcell_off; a.bface_id = L.bface_id;
-    a.cm = (const S *)L.cm; a.ncells = L.ncells;
-    const size_t sh = 2 * (size_t)CPB * c.D * sizeof(double);
-    hipLaunchKernelGGL((k_rhs<G, NC, R, S>), dim3((unsigned)nb), dim3(256), sh, st, a, c);
+    a.cm = (const double *)L.cm; a.ncells = L.ncells;
+    a.list = L.list; a.list_count = L.list_count;
+    const size_t sh = 2 * (size_t)CPB * c.D * sizeof(double) + 3 * (size_t)c.nf * sizeof(double)
+                      + (size_t)CPB * G * G * sizeof(int);
+    if (precision == CIPHER_FP32_MIXED) {
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, L.list_count);
+        hipLaunchKernelGGL((k_rhs<G, NC, float, false>), dim3((unsigned)nb), dim3(256), sh, st, a, c);
+        // FP64 recomputation of the cells whose flags sat in the FP32 band
+        const long long rb = nb < 512 ? nb : 512;
+        hipLaunchKernelGGL((k_rhs<G, NC, double, true>), dim3((unsigned)rb), dim3(256), sh, st, a, c);
+    } else {
+        hipLaunchKernelGGL((k_rhs<G, NC, double, false>), dim3((unsigned)nb), dim3(256), sh, st, a, c);
+    }
     return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
 }
 
-template <int NC, class R, class S>
-static cipher_status launch_NC(const RhsLaunch &L, hipStream_t st) {
+template <int NC>
+static cipher_status launch_NC(const RhsLaunch &L, int precision, hipStream_t st) {
     const int P = L.cfg->P;
-    if (P <= 4) return launch_G<4, NC, R, S>(L, st);
-    if (P <= 8) return launch_G<8, NC, R, S>(L, st);
-    if (P <= 16) return launch_G<16, NC, R, S>(L, st);
+    if (P <= 4) return launch_G<4, NC>(L, precision, st);
+    if (P <= 8) return launch_G<8, NC>(L, precision, st);
+    if (P <= 16) return launch_G<16, NC>(L, precision, st);
     return CIPHER_ERR_UNSUPPORTED;
 }
 
-template <class R, class S>
-static cipher_status launch_R(const RhsLaunch &L, hipStream_t st) {
+cipher_status launch_rhs(const RhsLaunch &L, int precision, hipStream_t st) {
     switch (L.cfg->ncp) {
-    case 1: return launch_NC<1, R, S>(L, st);
-    case 2: return launch_NC<2, R, S>(L, st);
-    case 3: return launch_NC<3, R, S>(L, st);
-    case 4: return launch_NC<4, R, S>(L, st);
-    case 5: return launch_NC<5, R, S>(L, st);
+    case 1: return launch_NC<1>(L, precision, st);
+    case 2: return launch_NC<2>(L, precision, st);
+    case 3: return launch_NC<3>(L, precision, st);
+    case 4: return launch_NC<4>(L, precision, st);
+    case 5: return launch_NC<5>(L, precision, st);
     default: return CIPHER_ERR_UNSUPPORTED;
     }
 }
 
-cipher_status launch_rhs(const RhsLaunch &L, int precision, hipStream_t st) {
-    if (precision == CIPHER_FP32_MIXED) return launch_R<float, float>(L, st);
-    return launch_R<double, double>(L, st);
-}
-
-size_t prep_record_bytes(int ncp, int precision) {
-    return (size_t)(2 * ncp + 1) * (precision == CIPHER_FP32_MIXED ? sizeof(float) : sizeof(double));
-}
+size_t prep_record_bytes(int ncp) { return (size_t)(2 * ncp + 1) * sizeof(double); }
 
 }  // namespace cipher
