@@ -49,6 +49,8 @@ struct cipher_ctx {
     int32_t *err = nullptr;
     double *partial = nullptr;
     double *norm = nullptr;
+    double *tsv = nullptr;          // folded T-series (ntseries + 1)
+    bool fold = false;
     double *h_norm = nullptr;
     int32_t *h_err = nullptr;
     int npart = 1024;
@@ -216,6 +218,18 @@ extern "C" int cipher_ctx_create(const cipher_config *cfg, int precision, cipher
         for (int s = 0; s < f.nsources; s++)
             if (cfg->source[f.source + s].model == CIPHER_SOURCE_SINK) d.has_sources = 1;
     }
+    d.ntseries = cfg->ntseries;
+    d.tsv = nullptr;
+    c->fold = (cfg->ncp > 1) || d.has_tcond || d.has_theat || d.has_iact;
+    if (hipMalloc(&c->tsv, sizeof(double) * (cfg->ntseries + 1)) != hipSuccess) {
+        cipher_ctx_destroy(c);
+        return fail(CIPHER_ERR_HIP, "scratch allocation failed");
+    }
+    {
+        std::vector<double> nanv(cfg->ntseries + 1, std::nan(""));
+        HIPCHK(hipMemcpy(c->tsv, nanv.data(), sizeof(double) * nanv.size(), hipMemcpyHostToDevice));
+    }
+    if (c->fold) d.tsv = c->tsv;
     if (hipMalloc(&c->err, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->list_count, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->partial, sizeof(double) * c->npart) != hipSuccess ||
         hipMalloc(&c->norm, sizeof(double)) != hipSuccess ||
         hipHostMalloc(&c->h_norm, sizeof(double), hipHostMallocDefault) != hipSuccess ||
@@ -235,6 +249,7 @@ extern "C" int cipher_ctx_destroy(cipher_ctx *c) {
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->err) (void)hipFree(c->err);
     if (c->list_count) (void)hipFree(c->list_count);
+    if (c->tsv) (void)hipFree(c->tsv);
     if (c->partial) (void)hipFree(c->partial);
     if (c->norm) (void)hipFree(c->norm);
     if (c->h_norm) (void)hipHostFree(c->h_norm);
@@ -337,6 +352,10 @@ static int rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active)
     L.list = c->list;
     L.list_count = c->list_count;
     L.ncells = c->ncells;
+    if (c->fold) {
+        cipher_status f = launch_tsfold(X, c->ncells, c->dc, c->tsv, c->stream);
+        if (f != CIPHER_OK) return fail(f, "fold launch");
+    }
     cipher_status s = launch_rhs(L, c->precision, c->stream);
     if (s != CIPHER_OK) return fail(s, std::string("rhs launch: ") + hipGetErrorString(hipGetLastError()));
     return CIPHER_OK;

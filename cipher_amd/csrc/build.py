@@ -9,7 +9,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
-SOURCES = ["rhs.hip", "post.hip", "ts.hip", "api.hip"]
+SOURCES = ["rhs_nc1.hip", "rhs_nc2.hip", "rhs_nc3.hip", "rhs_nc4.hip", "rhs_nc5.hip", "rhs.hip", "post.hip", "ts.hip", "api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -25,7 +25,7 @@ def _stale(src, obj):
     return any(os.path.getmtime(d) > t for d in deps)
 
 
-def build(verbose=False, jobs=4, force=False):
+def build(verbose=False, jobs=8, force=False):
     objdir = os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
     procs = []

@@ -38,6 +38,9 @@ struct DevCfg {
     int has_potential;  // some interface has a segregation potential
     int has_imobc;      // some interface has a solute mobility factor
     int has_iact;       // some interface has an energy/mobility activation
+    int ntseries;
+    // T-series values folded at one temperature: tsv[i] = SumTSeries(tsv[ntseries], ts[i])
+    const double *tsv;
 };
 
 template <class R> __device__ __forceinline__ R dexp(R x);
@@ -76,6 +79,14 @@ __device__ inline R dtseries(const cipher_tseries *t, R T) {
     return r;
 }
 
+// SumTSeries of table entry i; a cell at the folding temperature reads the
+// value the fold kernel computed with the same code (identical bits)
+template <class R>
+__device__ __forceinline__ R dts(const DevCfg &c, int i, R T) {
+    if (c.tsv && (double)T == c.tsv[c.ntseries]) return (R)c.tsv[i];
+    return dtseries<R>(&c.ts[i], T);
+}
+
 // SumTSeries_derivative, src/utility.c:67-79
 template <class R>
 __device__ inline R dtseries_deriv(const cipher_tseries *t, R T) {
@@ -87,6 +98,11 @@ __device__ inline R dtseries_deriv(const cipher_tseries *t, R T) {
     }
     if (fabs(t->logcoeff) > 0.0) r += (R)t->logcoeff * (R(1) + dlog(T));
     return r;
+}
+
+template <class R>
+__device__ __forceinline__ R dts_deriv(const DevCfg &c, int i, R T) {
+    return dtseries_deriv<R>(&c.ts[i], T);
 }
 
 // active-set count/id read from a double in the state, clamped so that a
@@ -170,19 +186,25 @@ __device__ __forceinline__ void dmatvec(R *dst, const R *a, const R *b) {
 }
 
 // element ix of a compile-time sized register array without dynamic indexing
+// (a sum of masked terms; the compiler turns a plain select chain back into
+// an indexed private-memory load)
 template <int N, class R>
 __device__ __forceinline__ R dsel(const R *a, int ix) {
-    R v = a[0];
+    R v = R(0);
 #pragma unroll
-    for (int c = 1; c < N; c++)
-        if (c == ix) v = a[c];
+    for (int c = 0; c < N; c++) {
+        const R m = (c == ix) ? R(1) : R(0);
+        v += m * a[c];
+    }
     return v;
 }
 template <int N, class R>
 __device__ __forceinline__ void daddat(R *a, int ix, R v) {
 #pragma unroll
-    for (int c = 0; c < N; c++)
-        if (c == ix) a[c] += v;
+    for (int c = 0; c < N; c++) {
+        const R m = (c == ix) ? R(1) : R(0);
+        a[c] += m * v;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -223,9 +245,9 @@ __device__ inline void d_sitefrac(const DevCfg &c, const cipher_material &m, R *
         R mt[Cnt<ND * ND>::v], mi[Cnt<ND * ND>::v];
 #pragma unroll
         for (int k = 0; k < NC; k++) {
-            ceq[k] = dtseries<R>(&c.ts[m.quad_ceq + k], T);
-            un[k] = dtseries<R>(&c.ts[m.unary + k], T);
-            bi[k] = dtseries<R>(&c.ts[m.quad_binary + k], T);
+            ceq[k] = dts<R>(c, m.quad_ceq + k, T);
+            un[k] = dts<R>(c, m.unary + k, T);
+            bi[k] = dts<R>(c, m.quad_binary + k, T);
         }
 #pragma unroll
         for (int k = 0; k < ND * ND; k++) { mt[k] = R(0); mi[k] = R(0); }
@@ -263,12 +285,12 @@ __device__ inline void d_mobility(const DevCfg &c, const cipher_material &m, R *
             R mig = R(0);
 #pragma unroll
             for (int j = 0; j < NC; j++) {
-                mig += y[j] * dtseries<R>(&c.ts[mb.unary + j], T);
+                mig += y[j] * dts<R>(c, mb.unary + j, T);
 #pragma unroll
                 for (int i = j + 1; i < NC; i++, bin++) {
                     const int nrk = bin->n;
                     for (int o = 0; o < nrk; o++)
-                        mig += y[j] * y[i] * dtseries<R>(&c.ts[bin->ts + o], T) * dfastpow(y[j] - y[i], (unsigned)o);
+                        mig += y[j] * y[i] * dts<R>(c, bin->ts + o, T) * dfastpow(y[j] - y[i], (unsigned)o);
                 }
             }
             mob[k] = y[k] * (R)mb.m0 / (R)CRG / T * dexp<R>(mig / (R)CRG / T);
@@ -290,7 +312,7 @@ __device__ inline R d_chemenergy(const DevCfg &c, const cipher_material &m, cons
 #pragma unroll
         for (int cp = 0; cp < NC; cp++)
 #pragma unroll
-            for (int cq = 0; cq < NC; cq++) e += dtseries<R>(&c.ts[m.unary + cp * NC + cq], T) * yp[cp] * yq[cq];
+            for (int cq = 0; cq < NC; cq++) e += dts<R>(c, m.unary + cp * NC + cq, T) * yp[cp] * yq[cq];
         const cipher_rk *rk = &c.rk[m.binary];
 #pragma unroll
         for (int i = 0; i < NC; i++)
@@ -299,7 +321,7 @@ __device__ inline R d_chemenergy(const DevCfg &c, const cipher_material &m, cons
 #pragma unroll
                 for (int cq = 0; cq < NC; cq++, rk++)
                     for (int o = 0; o < rk->n; o++)
-                        e += dtseries<R>(&c.ts[rk->ts + o], T) * yp[i] * yp[j] * yq[cq] * dfastpow(yp[i] - yp[j], (unsigned)o);
+                        e += dts<R>(c, rk->ts + o, T) * yp[i] * yp[j] * yq[cq] * dfastpow(yp[i] - yp[j], (unsigned)o);
         rk = &c.rk[m.binaryq];
 #pragma unroll
         for (int cp = 0; cp < NC; cp++)
@@ -308,7 +330,7 @@ __device__ inline R d_chemenergy(const DevCfg &c, const cipher_material &m, cons
 #pragma unroll
                 for (int j = i + 1; j < NC; j++, rk++)
                     for (int o = 0; o < rk->n; o++)
-                        e += dtseries<R>(&c.ts[rk->ts + o], T) * yp[cp] * yq[i] * yq[j] * dfastpow(yq[i] - yq[j], (unsigned)o);
+                        e += dts<R>(c, rk->ts + o, T) * yp[cp] * yq[i] * yq[j] * dfastpow(yq[i] - yq[j], (unsigned)o);
         rk = &c.rk[m.ternary];
 #pragma unroll
         for (int i = 0; i < NC; i++)
@@ -319,7 +341,7 @@ __device__ inline R d_chemenergy(const DevCfg &c, const cipher_material &m, cons
 #pragma unroll
                     for (int cq = 0; cq < NC; cq++, rk++)
                         for (int o = 0; o < rk->n; o++)
-                            e += dtseries<R>(&c.ts[rk->ts + o], T) * yp[i] * yp[j] * yp[k] * yq[cq];
+                            e += dts<R>(c, rk->ts + o, T) * yp[i] * yp[j] * yp[k] * yq[cq];
         rk = &c.rk[m.ternaryq];
 #pragma unroll
         for (int cp = 0; cp < NC; cp++)
@@ -330,37 +352,37 @@ __device__ inline R d_chemenergy(const DevCfg &c, const cipher_material &m, cons
 #pragma unroll
                     for (int k = j + 1; k < NC; k++, rk++)
                         for (int o = 0; o < rk->n; o++)
-                            e += dtseries<R>(&c.ts[rk->ts + o], T) * yq[i] * yq[j] * yq[k] * yp[cp];
+                            e += dts<R>(c, rk->ts + o, T) * yq[i] * yq[j] * yq[k] * yp[cp];
         return e;
     } else if (m.model == CIPHER_CHEM_CALPHADDIS) {
         const cipher_rk *bin = &c.rk[m.binary], *ter = &c.rk[m.ternary];
-        R e = dtseries<R>(&c.ts[m.ref], T);
+        R e = dts<R>(c, m.ref, T);
 #pragma unroll
         for (int k = 0; k < NC; k++) {
-            e += dtseries<R>(&c.ts[m.unary + k], T) * y[k] + (R)CRG * T * y[k] * dlog(y[k]);
+            e += dts<R>(c, m.unary + k, T) * y[k] + (R)CRG * T * y[k] * dlog(y[k]);
 #pragma unroll
             for (int j = k + 1; j < NC; j++, bin++) {
                 for (int o = 0; o < bin->n; o++)
-                    e += dtseries<R>(&c.ts[bin->ts + o], T) * y[k] * y[j] * dfastpow(y[k] - y[j], (unsigned)o);
+                    e += dts<R>(c, bin->ts + o, T) * y[k] * y[j] * dfastpow(y[k] - y[j], (unsigned)o);
 #pragma unroll
                 for (int i = j + 1; i < NC; i++, ter++)
                     for (int o = 0; o < ter->n; o++)
-                        e += dtseries<R>(&c.ts[ter->ts + o], T) * y[k] * y[j] * y[i]
+                        e += dts<R>(c, ter->ts + o, T) * y[k] * y[j] * y[i]
                              * (dsel<NC, R>(y, c.rkindex[ter->idx + o]) + (R(1) - y[k] - y[j] - y[i]) / R(3));
             }
         }
         return e;
     } else if (m.model == CIPHER_CHEM_QUADRATIC) {
-        R e = dtseries<R>(&c.ts[m.ref], T);
+        R e = dts<R>(c, m.ref, T);
 #pragma unroll
         for (int k = 0; k < NC; k++) {
-            const R ceq = dtseries<R>(&c.ts[m.quad_ceq + k], T);
-            e += dtseries<R>(&c.ts[m.unary + k], T) * (y[k] - ceq)
-                 + dtseries<R>(&c.ts[m.quad_binary + k], T) * (y[k] - ceq) * (y[k] - ceq);
+            const R ceq = dts<R>(c, m.quad_ceq + k, T);
+            e += dts<R>(c, m.unary + k, T) * (y[k] - ceq)
+                 + dts<R>(c, m.quad_binary + k, T) * (y[k] - ceq) * (y[k] - ceq);
         }
         return e;
     }
-    return dtseries<R>(&c.ts[m.ref], T);
+    return dts<R>(c, m.ref, T);
 }
 
 // SitepotentialExplicit, src/material.c:495-650: out nsites*(NC-1)
@@ -378,7 +400,7 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
         for (int cp = 0; cp < NC; cp++)
 #pragma unroll
             for (int cq = 0; cq < NC; cq++) {
-                a = dtseries<R>(&c.ts[m.unary + cp * NC + cq], T);
+                a = dts<R>(c, m.unary + cp * NC + cq, T);
                 pp[cp] += a * yq[cq];
                 pq[cq] += a * yp[cp];
             }
@@ -390,7 +412,7 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
 #pragma unroll
                 for (int cq = 0; cq < NC; cq++, rk++)
                     for (int o = 0; o < rk->n; o++) {
-                        a = dtseries<R>(&c.ts[rk->ts + o], T);
+                        a = dts<R>(c, rk->ts + o, T);
                         b = dfastpow(yp[i] - yp[j], (unsigned)o);
                         cc = (o == 0) ? R(0) : ((R)o) * dfastpow(yp[i] - yp[j], (unsigned)(o - 1));
                         pp[i] += a * yp[j] * yq[cq] * (b + cc * yp[i]);
@@ -405,7 +427,7 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
 #pragma unroll
                 for (int j = i + 1; j < NC; j++, rk++)
                     for (int o = 0; o < rk->n; o++) {
-                        a = dtseries<R>(&c.ts[rk->ts + o], T);
+                        a = dts<R>(c, rk->ts + o, T);
                         b = dfastpow(yq[i] - yq[j], (unsigned)o);
                         cc = (o == 0) ? R(0) : ((R)o) * dfastpow(yq[i] - yq[j], (unsigned)(o - 1));
                         pq[i] += a * yq[j] * yp[cp] * (b + cc * yq[i]);
@@ -422,7 +444,7 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
 #pragma unroll
                     for (int cq = 0; cq < NC; cq++, rk++)
                         for (int o = 0; o < rk->n; o++) {
-                            a = dtseries<R>(&c.ts[rk->ts + o], T);
+                            a = dts<R>(c, rk->ts + o, T);
                             pp[i] += a * yp[j] * yp[k] * yq[cq];
                             pp[j] += a * yp[k] * yp[i] * yq[cq];
                             pp[k] += a * yp[i] * yp[j] * yq[cq];
@@ -438,7 +460,7 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
 #pragma unroll
                     for (int k = j + 1; k < NC; k++, rk++)
                         for (int o = 0; o < rk->n; o++) {
-                            a = dtseries<R>(&c.ts[rk->ts + o], T);
+                            a = dts<R>(c, rk->ts + o, T);
                             pq[i] += a * yq[j] * yq[k] * yp[cp];
                             pq[j] += a * yq[k] * yq[i] * yp[cp];
                             pq[k] += a * yq[i] * yq[j] * yp[cp];
@@ -447,14 +469,14 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
     } else if (m.model == CIPHER_CHEM_CALPHADDIS) {
         R a, b, cc;
 #pragma unroll
-        for (int i = 0; i < NC; i++) full[i] = dtseries<R>(&c.ts[m.unary + i], T);
+        for (int i = 0; i < NC; i++) full[i] = dts<R>(c, m.unary + i, T);
         const cipher_rk *rk = &c.rk[m.binary];
 #pragma unroll
         for (int i = 0; i < NC; i++)
 #pragma unroll
             for (int j = i + 1; j < NC; j++, rk++)
                 for (int o = 0; o < rk->n; o++) {
-                    a = dtseries<R>(&c.ts[rk->ts + o], T);
+                    a = dts<R>(c, rk->ts + o, T);
                     b = dfastpow(y[i] - y[j], (unsigned)o);
                     cc = (o == 0) ? R(0) : ((R)o) * dfastpow(y[i] - y[j], (unsigned)(o - 1));
                     full[i] += a * y[j] * (b + cc * y[i]);
@@ -469,7 +491,7 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
                 for (int k = j + 1; k < NC; k++, rk++)
                     for (int o = 0; o < rk->n; o++) {
                         const int ix = c.rkindex[rk->idx + o];
-                        a = dtseries<R>(&c.ts[rk->ts + o], T);
+                        a = dts<R>(c, rk->ts + o, T);
                         b = dsel<NC, R>(y, ix) + (R(1) - y[k] - y[j] - y[i]) / R(3);
                         full[i] += a * y[j] * y[k] * (b - y[i] / R(3));
                         full[j] += a * y[k] * y[i] * (b - y[j] / R(3));
@@ -505,8 +527,8 @@ __device__ inline void d_sitepot_implicit(const DevCfg &c, const cipher_material
     } else if (m.model == CIPHER_CHEM_QUADRATIC) {
 #pragma unroll
         for (int k = 0; k < NC; k++)
-            full[k] = dtseries<R>(&c.ts[m.unary + k], T)
-                      + R(2) * dtseries<R>(&c.ts[m.quad_binary + k], T) * (y[k] - dtseries<R>(&c.ts[m.quad_ceq + k], T));
+            full[k] = dts<R>(c, m.unary + k, T)
+                      + R(2) * dts<R>(c, m.quad_binary + k, T) * (y[k] - dts<R>(c, m.quad_ceq + k, T));
     }
 #pragma unroll
     for (int s = 0; s < 2; s++) {
@@ -562,11 +584,11 @@ __device__ inline void d_sitefrac_tangent(const DevCfg &c, const cipher_material
         R mt[Cnt<NN>::v], mi[Cnt<NN>::v];
 #pragma unroll
         for (int k = 0; k < NC; k++) {
-            ceqT[k] = dtseries_deriv<R>(&c.ts[m.quad_ceq + k], T);
-            unT[k] = dtseries_deriv<R>(&c.ts[m.unary + k], T);
-            biT[k] = dtseries_deriv<R>(&c.ts[m.quad_binary + k], T);
-            ceq[k] = dtseries<R>(&c.ts[m.quad_ceq + k], T);
-            bi[k] = dtseries<R>(&c.ts[m.quad_binary + k], T);
+            ceqT[k] = dts_deriv<R>(c, m.quad_ceq + k, T);
+            unT[k] = dts_deriv<R>(c, m.unary + k, T);
+            biT[k] = dts_deriv<R>(c, m.quad_binary + k, T);
+            ceq[k] = dts<R>(c, m.quad_ceq + k, T);
+            bi[k] = dts<R>(c, m.quad_binary + k, T);
         }
 #pragma unroll
         for (int k = 0; k < NN; k++) { mt[k] = R(0); mi[k] = R(0); }

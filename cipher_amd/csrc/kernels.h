@@ -35,6 +35,7 @@ struct RhsLaunch {
 
 cipher_status launch_rhs(const RhsLaunch &L, int precision, hipStream_t st);
 size_t prep_record_bytes(int ncp);
+cipher_status launch_tsfold(const double *X, long long ncells, const DevCfg &c, double *tsv, hipStream_t st);
 
 struct PostLaunch {
     const DevCfg *cfg;

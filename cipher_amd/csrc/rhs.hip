@@ -1,931 +1,12 @@
-// rhs.hip - the phase-field right-hand side on gfx950.
-//
-// Restates RHSFunction of micmog/CIPHER (src/cipher.c:27-681) as two kernels:
-//
-//   k_prep  per-cell composition, volume-ref mobility and thermal
-//           conductivity (src/cipher.c:149-233) into a compact FP64 record,
-//           only when the face fluxes need them (chemistry or conduction);
-//   k_rhs   per-cell gather over the cell's faces in face order (the
-//           reference scatters faces in the same order, so each cell sums its
-//           Laplacian contributions in the same sequence, src/cipher.c:236-389)
-//           fused with the cell update (src/cipher.c:392-664).
-//
-// Mapping: G lanes per cell (G = next power of two >= PF_SIZE), lane g owns
-// active slot g.  Per-cell sums over slots are replayed by every lane in the
-// reference's loop order through group shuffles, so FP64 results follow the
-// reference's rounding sequence.  A block's contiguous X records are staged in
-// LDS with coalesced loads and its F records are assembled in LDS and stored
-// coalesced.  Neighbour records are fetched a batch of faces at a time so the
-// dependent loads of several faces are in flight together.
-//
-// FP32-mixed: the face fluxes (Kahan-compensated), interpolants, capillary and
-// rate assembly run in FP32; the implicit site potentials, site fractions,
-// chemical energy, explicit potentials and their rate stay FP64.  Every slot
-// whose active flag is decided by the sign of an FP32 value lying inside its
-// rounding band is recomputed by the FP64 kernel (LIST mode), so the active
-// sets equal the FP64 path's.
+// rhs.hip - dispatch of the right-hand side on the number of components
+// (the kernels live in rhs_impl.h, instantiated once per component count in
+// rhs_nc1.hip .. rhs_nc5.hip).
 #include "kernels.h"
 
 namespace cipher {
 
-template <int G, class T>
-__device__ __forceinline__ T gsh(T v, int src) {
-    return __shfl(v, src, G);
-}
-
-// interface energy of an interface type at T (src/cipher.c:425, 462-465)
-template <class R>
-__device__ __forceinline__ R itf_energy(const DevCfg &c, int it, R T) {
-    const cipher_interface &f = c.itf[it];
-    R e = R(1);
-    e *= (R)f.e0;
-    if (c.has_iact && c.ts[f.e_act].n) e *= dexp<R>(dtseries<R>(&c.ts[f.e_act], T) / (R)CRG / T);
-    return e;
-}
-// interface mobility of an interface type at T (src/cipher.c:426, 466-469)
-template <class R>
-__device__ __forceinline__ R itf_mobility(const DevCfg &c, int it, R T) {
-    const cipher_interface &f = c.itf[it];
-    R m = R(1);
-    m *= (R)f.m0;
-    if (c.has_iact && c.ts[f.m_act].n) m *= dexp<R>(dtseries<R>(&c.ts[f.m_act], T) / (R)CRG / T);
-    return m;
-}
-
-// Kahan-compensated accumulator in FP32; plain sum (reference order) in FP64
-template <class R>
-struct Acc {
-    R s = R(0), comp = R(0);
-    __device__ __forceinline__ void add(R v) {
-        if constexpr (sizeof(R) == 4) {
-            R y = v - comp;
-            R t = s + y;
-            comp = (t - s) - y;
-            s = t;
-        } else {
-            s += v;
-        }
-    }
-};
-
-// per-cell interpolants of the active slots (src/utility.c:470-535)
-template <int G, class R>
-__device__ __forceinline__ void interpolant(const DevCfg &c, int n, R phi, R &hh, R &h) {
-    hh = (c.interp == CIPHER_INTERP_QUADRATIC) ? phi * phi : phi * phi * (R(3) - R(2) * phi);
-    R hsum = R(0);
-    for (int g = 0; g < n; g++) hsum += gsh<G>(hh, g);
-    if (c.interp == CIPHER_INTERP_LINEAR) h = phi;
-    else if (n == 1) h = R(1);
-    else h = hh / hsum;
-}
-
-// in <- dh/dphi^T in for the slot-distributed vector (src/utility.c:478-557)
-template <int G, class R>
-__device__ __forceinline__ R interp_deriv(const DevCfg &c, int n, R phi, R hh, R in) {
-    if (c.interp == CIPHER_INTERP_LINEAR) return in;
-    if (n == 1) return R(0);
-    R avg = R(0), sum = R(0);
-    for (int g = 0; g < n; g++) {
-        const R hg = gsh<G>(hh, g);
-        const R ig = gsh<G>(in, g);
-        sum += hg;
-        avg += hg * ig;
-    }
-    if (c.interp == CIPHER_INTERP_QUADRATIC) return R(2) * phi * (in - avg / sum) / sum;
-    return R(6) * phi * (R(1) - phi) * (in - avg / sum) / sum;
-}
-
-// segregation-potential interface sum mu_int[c] = sum_{k<j} h_k h_j V_kj[c]
-template <int G, int NC, class R>
-__device__ __forceinline__ void interface_mu(const DevCfg &c, int n, int myid, R h, R *mui) {
-    constexpr int ND = NC - 1;
-#pragma unroll
-    for (int k = 0; k < Cnt<ND>::v; k++) mui[k] = R(0);
-    if (!c.has_potential) return;
-    for (int gk = 0; gk < n; gk++) {
-        const int idk = gsh<G>(myid, gk);
-        const R hk = gsh<G>(h, gk);
-        for (int gj = gk + 1; gj < n; gj++) {
-            const int idj = gsh<G>(myid, gj);
-            const R hj = gsh<G>(h, gj);
-            const cipher_interface &f = c.itf[c.ilist[idk * c.npf + idj]];
-            if (f.has_potential) {
-#pragma unroll
-                for (int k = 0; k < ND; k++) mui[k] += hk * hj * (R)f.potential[k];
-            }
-        }
-    }
-}
-
-// implicit site potentials and site fractions of the slot's phase, in FP64
-// (src/cipher.c:170-189 and 584-609): spim = nu_s (mu - mu_int) - EX
-template <int G, int NC>
-__device__ __forceinline__ void slot_sitefrac(const DevCfg &c, const cipher_material &m, bool on, int ns, int n,
-                                              int myid, double h, const double *x, int slot, double T, double *spim,
-                                              double *y) {
-    constexpr int ND = NC - 1;
-    double mui[Cnt<ND>::v];
-    interface_mu<G, NC, double>(c, n, myid, h, mui);
-#pragma unroll
-    for (int s = 0; s < 2; s++)
-#pragma unroll
-        for (int k = 0; k < ND; k++)
-            spim[s * ND + k] = (s < ns) ? m.stoichiometry[s] * (x[c.dpo + k] - mui[k]) - x[c.exo + slot * c.SP + s * ND + k]
-                                        : 0.0;
-#pragma unroll
-    for (int k = 0; k < 2 * NC; k++) y[k] = 0.0;
-    if (on) d_sitefrac<NC, double>(c, m, y, spim, T);
-}
-
-// ---------------------------------------------------------------------------
-// block staging of AoS records: contiguous (block of consecutive cells) or,
-// in LIST mode, each group copies its own cell's record
-__device__ __forceinline__ void stage_in(double *lds, const double *X, long long c0, int ncell_blk, int D) {
-    const long long base = c0 * D;
-    const int tot = ncell_blk * D;
-    for (int i = threadIdx.x; i < tot; i += blockDim.x) lds[i] = X[base + i];
-}
-__device__ __forceinline__ void stage_out(double *F, const double *lds, long long c0, int ncell_blk, int D) {
-    const long long base = c0 * D;
-    const int tot = ncell_blk * D;
-    for (int i = threadIdx.x; i < tot; i += blockDim.x) F[base + i] = lds[i];
-}
-
-// ---------------------------------------------------------------------------
-// k_prep: composition, mobility, conductivity per cell (src/cipher.c:149-233)
-template <int G, int NC>
-__global__ __launch_bounds__(256) void k_prep(const double *__restrict__ X, double *__restrict__ cm,
-                                              long long ncells, DevCfg c) {
-    constexpr int ND = NC - 1;
-    constexpr int CPB = 256 / G;
-    extern __shared__ double lds[];
-    const long long c0 = (long long)blockIdx.x * CPB;
-    const int nblk = (int)min((long long)CPB, ncells - c0);
-    stage_in(lds, X, c0, nblk, c.D);
-    __syncthreads();
-    const int lc = threadIdx.x / G;
-    const int slot = threadIdx.x & (G - 1);
-    if (lc >= nblk) return;
-    const long long cell = c0 + lc;
-    const double *x = lds + lc * c.D;
-    const int n = clamp_count(x[c.aso], c.P);
-    const bool on = slot < n;
-    const int myid = on ? clamp_id(x[c.aso + 1 + slot], c.npf) : 0;
-    const double phi = on ? x[c.pfo + slot] : 0.0;
-    const double T = x[c.tmo];
-    double hh, h;
-    interpolant<G, double>(c, n, phi, hh, h);
-    if (!on) h = 0.0;
-    const int mat = c.phase_material[myid];
-    const cipher_material &m = c.mat[mat];
-    double comp[NC], mob[NC];
-    double tcond = 0.0;
-    if constexpr (NC > 1) {
-        double spim[2 * Cnt<ND>::v], y[2 * NC], me[NC];
-        const int ns = on ? m.nsites : 0;
-        slot_sitefrac<G, NC>(c, m, on, ns, n, myid, h, x, slot, T, spim, y);
-        if (on) {
-            d_mobility<NC, double>(c, m, me, y, T);
-        } else {
-#pragma unroll
-            for (int k = 0; k < NC; k++) me[k] = 0.0;
-        }
-        // per-slot terms h*me and h*nu_s*y, summed in slot order
-#pragma unroll
-        for (int k = 0; k < NC; k++) { comp[k] = 0.0; mob[k] = 0.0; }
-        for (int g = 0; g < n; g++) {
-            const int nsg = gsh<G>(ns, g);
-#pragma unroll
-            for (int k = 0; k < NC; k++) mob[k] += gsh<G>(h * me[k], g);
-#pragma unroll
-            for (int s = 0; s < 2; s++) {
-                const double st = on && s < ns ? m.stoichiometry[s] : 0.0;
-#pragma unroll
-                for (int k = 0; k < NC; k++) {
-                    const double v = gsh<G>(h * st * y[s * NC + k], g);
-                    if (s < nsg) comp[k] += v;
-                }
-            }
-        }
-        if (c.has_imobc) {
-            for (int gk = 0; gk < n; gk++) {
-                const int idk = gsh<G>(myid, gk);
-                for (int gj = gk + 1; gj < n; gj++) {
-                    const int idj = gsh<G>(myid, gj);
-                    const cipher_interface &f = c.itf[c.ilist[idk * c.npf + idj]];
-                    if (f.has_mobilityc) {
-#pragma unroll
-                        for (int k = 0; k < NC; k++) mob[k] *= exp(f.mobilityc[k] / CRG / T);
-                    }
-                }
-            }
-        }
-    }
-    if (c.has_tcond) {
-        const double tv = (on && c.ts[m.tconductivity].n) ? h * dtseries<double>(&c.ts[m.tconductivity], T) : 0.0;
-        for (int g = 0; g < n; g++) {
-            const int idg = gsh<G>(myid, g);
-            const double v = gsh<G>(tv, g);
-            if (c.ts[c.mat[c.phase_material[idg]].tconductivity].n) tcond += v;
-        }
-    }
-    if (slot == 0) {
-        double *o = cm + cell * (2 * NC + 1);
-        if constexpr (NC > 1) {
-#pragma unroll
-            for (int k = 0; k < NC; k++) { o[k] = comp[k]; o[NC + k] = mob[k]; }
-        }
-        o[2 * NC] = tcond;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// neighbour data of one face, fetched ahead of its use
 template <int NC>
-struct FaceIn {
-    int e;
-    double h, n, id0, id, ph, T;
-    double cm[NC > 1 ? 2 * NC + 1 : 1];
-    double mu[Cnt<NC - 1>::v];
-};
-
-template <int NC>
-__device__ __forceinline__ void load_face(FaceIn<NC> &f, const RhsArgs &a, const DevCfg &c, int e, int slot) {
-    constexpr int ND = NC - 1;
-    f.e = e;
-    const long long nb = e >> 1;
-    const double *xn = a.X + nb * c.D;
-    f.h = a.cellgeom[nb];
-    f.n = xn[c.aso];
-    f.id0 = xn[c.aso + 1];
-    f.id = slot < c.P ? xn[c.aso + 1 + slot] : 0.0;
-    f.ph = slot < c.P ? xn[c.pfo + slot] : 0.0;
-    if constexpr (NC > 1) {
-#pragma unroll
-        for (int q = 0; q < 2 * NC + 1; q++) f.cm[q] = a.cm[nb * (2 * NC + 1) + q];
-#pragma unroll
-        for (int q = 0; q < ND; q++) f.mu[q] = xn[c.dpo + q];
-    }
-    if (c.has_tcond) {
-        f.T = xn[c.tmo];
-        if constexpr (NC == 1) f.cm[0] = a.cm[nb * (2 * NC + 1) + 2 * NC];
-    }
-}
-
-// interface constants staged in LDS: [e0, width, m0] per interface type
-template <class R>
-__device__ __forceinline__ R pair_e(const DevCfg &c, const double *itab, int it, R T) {
-    if (c.has_iact) return itf_energy<R>(c, it, T);
-    R e = R(1);
-    e *= (R)itab[3 * it];
-    return e;
-}
-template <class R>
-__device__ __forceinline__ R pair_m(const DevCfg &c, const double *itab, int it, R T) {
-    if (c.has_iact) return itf_mobility<R>(c, it, T);
-    R m = R(1);
-    m *= (R)itab[3 * it + 2];
-    return m;
-}
-
-// make LDS writes of this wave visible to its other lanes
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// ---------------------------------------------------------------------------
-// k_rhs: fused face gather + cell right-hand side.  LIST: process the cells
-// of a.list[0 .. *a.list_count) (the FP64 recomputation of FP32-mixed cells
-// whose active flags were inside the rounding band).
-template <int G, int NC, class R, bool LIST>
-__global__ __launch_bounds__(256) void k_rhs(RhsArgs a, DevCfg c) {
-    constexpr int ND = NC - 1;
-    constexpr int NDV = Cnt<ND>::v;
-    constexpr int CPB = 256 / G;
-    constexpr bool MIXED = sizeof(R) == 4;
-    extern __shared__ double lds[];
-    double *ldsX = lds;
-    double *ldsF = lds + CPB * c.D;
-    double *itab = ldsF + CPB * c.D;                       // 3*nf doubles
-    int *ptype = (int *)(itab + 3 * c.nf);                 // CPB*G*G ints
-    const int lc = threadIdx.x / G;
-    const int slot = threadIdx.x & (G - 1);
-    const int D = c.D;
-    for (int i = threadIdx.x; i < c.nf; i += blockDim.x) {
-        itab[3 * i] = c.itf[i].e0;
-        itab[3 * i + 1] = c.itf[i].width;
-        itab[3 * i + 2] = c.itf[i].m0;
-    }
-    int *pt = ptype + lc * G * G;
-    const long long limit = LIST ? (long long)*a.list_count : a.ncells;
-    const long long stride = (long long)gridDim.x * CPB;
-    for (long long c0 = (long long)blockIdx.x * CPB; c0 < limit; c0 += stride) {
-    const int nblk = (int)min((long long)CPB, limit - c0);
-    long long cell = -1;
-    if (lc < nblk) cell = LIST ? (long long)a.list[c0 + lc] : c0 + lc;
-    if (LIST) {
-        __syncthreads();
-        if (lc < nblk)
-            for (int k = slot; k < D; k += G) ldsX[lc * D + k] = a.X[cell * D + k];
-    } else {
-        stage_in(ldsX, a.X, c0, nblk, D);
-    }
-    for (int i = threadIdx.x; i < nblk * D; i += blockDim.x) ldsF[i] = 0.0;
-    __syncthreads();
-    if (lc < nblk) {
-        const double *x = ldsX + lc * D;
-        double *fr = ldsF + lc * D;
-        const int n = clamp_count(x[c.aso], c.P);
-        const bool on = slot < n;
-        const int myid = on ? clamp_id(x[c.aso + 1 + slot], c.npf) : 0xFFFF;
-        const int id0 = clamp_id(x[c.aso + 1], c.npf);
-        const double phid = on ? x[c.pfo + slot] : 0.0;
-        const R phi = (R)phid;
-        const double Td = x[c.tmo];
-        const R T = (R)Td;
-        const double hS = a.cellgeom[cell];
-        const int dim = c.dim;
-
-        // ---- faces (src/cipher.c:236-317), in face order --------------------
-        Acc<R> lap;
-        Acc<R> dlap[NDV];
-        Acc<R> tlap;
-        R lapabs = R(0);
-        double cmS[2 * NC + 1];
-        if (NC > 1 || c.has_tcond) {
-#pragma unroll
-            for (int q = 0; q < 2 * NC + 1; q++) cmS[q] = a.cm[cell * (2 * NC + 1) + q];
-        }
-        const int kb = a.cell_off[cell], ke = a.cell_off[cell + 1];
-        // adjacency of the first 2G faces, one entry per lane
-        const int adj0 = (kb + slot < ke) ? a.adj[kb + slot] : 0;
-        const int adj1 = (kb + G + slot < ke) ? a.adj[kb + G + slot] : 0;
-        FaceIn<NC> cur, nxt;
-        if (kb < ke) load_face<NC>(cur, a, c, gsh<G>(adj0, 0), slot);
-        for (int k = kb; k < ke; k++) {
-            if (k + 1 < ke) {
-                const int r = k + 1 - kb;
-                const int e1 = r < G ? gsh<G>(adj0, r) : (r < 2 * G ? gsh<G>(adj1, r - G) : a.adj[k + 1]);
-                load_face<NC>(nxt, a, c, e1, slot);
-            }
-            {
-                const bool selfL = (cur.e & 1) == 0;
-                const double hN = cur.h;
-                const double dL = selfL ? hS : hN, dR = selfL ? hN : hS;
-                const double vL = dfastpow(dL, (unsigned)dim), vR = dfastpow(dR, (unsigned)dim);
-                const double ffd = 2.0 * (dL < dR ? vL / dL : vR / dR) / (dL + dR);
-                const R ff = (R)ffd, volL = (R)vL, volR = (R)vR;
-                const int nbn = clamp_count(cur.n, c.P);
-                const int Ln = selfL ? n : nbn;
-                const int Lid0 = selfL ? id0 : clamp_id(cur.id0, c.npf);
-                if (Ln > 1 || Lid0 > 1) {
-                    const int nbid = slot < nbn ? clamp_id(cur.id, c.npf) : 0xFFFE;
-                    const R nbphi = slot < nbn ? (R)cur.ph : R(0);
-                    int pos = -1;
-                    for (int t = 0; t < nbn; t++) {
-                        const int v = gsh<G>(nbid, t);
-                        if (v == myid) pos = t;
-                    }
-                    const R pv = gsh<G>(nbphi, pos < 0 ? 0 : pos);
-                    if (on && pos >= 0) {
-                        R term;
-                        if (selfL) {
-                            const R fl = pv - phi;
-                            term = ff * fl / volL;
-                        } else {
-                            const R fl = phi - pv;
-                            term = -(ff * fl / volR);
-                        }
-                        lap.add(term);
-                        if constexpr (MIXED) lapabs += fabsf(term);
-                    }
-                }
-                if constexpr (NC > 1) {
-                    const double *cmL = selfL ? cmS : cur.cm, *cmR = selfL ? cur.cm : cmS;
-                    R me[NC], ca[NC], MB[ND * ND], dmu[ND], fd[ND];
-#pragma unroll
-                    for (int q = 0; q < NC; q++) {
-                        const double mLd = cmL[NC + q], mRd = cmR[NC + q];
-                        const R mL = (R)mLd, mR = (R)mRd;
-                        if (fabs(mLd) > 1e-32 && fabs(mRd) > 1e-32) {
-                            if constexpr (MIXED) me[q] = R(2) * mL * (mR / (mL + mR));   // no FP32 underflow
-                            else me[q] = R(2) * mL * mR / (mL + mR);
-                        } else {
-                            me[q] = R(0);
-                        }
-                        const R cL = (R)cmL[q], cR = (R)cmR[q];
-                        ca[q] = R(2) * cL * cR / (cL + cR);
-                    }
-#pragma unroll
-                    for (int i = 0; i < ND; i++)
-#pragma unroll
-                        for (int jj = 0; jj < ND; jj++) {
-                            R v = R(0);
-#pragma unroll
-                            for (int q = 0; q < NC; q++)
-                                v += ((jj == q ? R(1) : R(0)) - ca[jj]) * ((q == i ? R(1) : R(0)) - ca[i]) * me[q];
-                            MB[i * ND + jj] = v;
-                        }
-#pragma unroll
-                    for (int q = 0; q < ND; q++) {
-                        const double mus = x[c.dpo + q];
-                        dmu[q] = selfL ? (R)(cur.mu[q] - mus) : (R)(mus - cur.mu[q]);
-                    }
-                    dmatvec<ND, R>(fd, MB, dmu);
-#pragma unroll
-                    for (int q = 0; q < ND; q++) {
-                        if (selfL) dlap[q].add(ff * fd[q] / volL);
-                        else dlap[q].add(-(ff * fd[q] / volR));
-                    }
-                }
-                if (c.has_tcond) {
-                    const double tSd = cmS[2 * NC];
-                    const double tNd = cur.cm[NC > 1 ? 2 * NC : 0];
-                    const double tLd = selfL ? tSd : tNd, tRd = selfL ? tNd : tSd;
-                    if (fabs(tLd) > 1e-32 && fabs(tRd) > 1e-32) {
-                        const R tL = (R)tLd, tR = (R)tRd;
-                        const double TL = selfL ? Td : cur.T, TR = selfL ? cur.T : Td;
-                        const R ft = R(2) * tL * tR / (tL + tR) * (R)(TR - TL);
-                        if (selfL) tlap.add(ff * ft / volL);
-                        else tlap.add(-(ff * ft / volR));
-                    }
-                }
-            }
-            if (k + 1 < ke) cur = nxt;
-        }
-        // ---- boundary faces (src/cipher.c:320-389), per condition ----------
-        if (c.nbcs > 0) {
-            const int bb = a.bcell_off[cell], be = a.bcell_off[cell + 1];
-            for (int b = 0; b < c.nbcs; b++) {
-                const cipher_bc &bc = c.bc[b];
-                for (int k = bb; k < be; k++) {
-                    if (a.bface_id[k] != bc.boundary_id) continue;
-                    const R dL = (R)hS;
-                    R fd[NDV];
-                    R ft = R(0);
-                    if (bc.chem_type != CIPHER_BC_NONE) {
-#pragma unroll
-                        for (int q = 0; q < ND; q++) {
-                            fd[q] = R(0);
-                            if (bc.chem_mask & (1 << q)) {
-                                if (bc.chem_type == CIPHER_BC_NEUMANN) fd[q] = (R)bc.chem_val[q] * dL;
-                                else fd[q] = (R)cmS[NC + q] * ((R)bc.chem_val[q] - (R)x[c.dpo + q]);
-                            }
-                        }
-                    }
-                    const R tc = c.has_tcond ? (R)cmS[2 * NC] : R(0);
-                    if (bc.thermal_type == CIPHER_BC_NEUMANN) ft = tc * (R)bc.thermal_val * dL;
-                    else if (bc.thermal_type == CIPHER_BC_DIRICHLET) ft = tc * ((R)bc.thermal_val - T);
-                    const R cf = R(1) / dfastpow(dL, 2u);
-                    if (bc.chem_type != CIPHER_BC_NONE) {
-#pragma unroll
-                        for (int q = 0; q < ND; q++) dlap[q].add(cf * fd[q]);
-                    }
-                    if (bc.thermal_type != CIPHER_BC_NONE) tlap.add(cf * ft);
-                }
-            }
-        }
-        const R plap = lap.s;
-
-        // ---- cell (src/cipher.c:392-664) ------------------------------------
-        R hh, h;
-        interpolant<G, R>(c, n, phi, hh, h);
-        if (!on) { h = R(0); hh = R(0); }
-        const int mat = on ? c.phase_material[myid] : 0;
-        const cipher_material &m = c.mat[mat];
-        const int ns = on ? m.nsites : 0;
-
-        // site fractions of the slot's phase, FP64 (same arithmetic as k_prep)
-        double yd[2 * NC], spimd[2 * NDV];
-        slot_sitefrac<G, NC>(c, m, on, ns, n, myid, (double)h, x, slot, Td, spimd, yd);
-        R y[2 * NC];
-#pragma unroll
-        for (int k = 0; k < 2 * NC; k++) y[k] = (R)yd[k];
-
-        // interface type of every active pair, row gk of the group's table
-        {
-            int row[G];
-#pragma unroll
-            for (int gj = 0; gj < G; gj++) {
-                const int idj = gsh<G>(myid, gj);
-                row[gj] = (on && gj > slot && gj < n) ? c.ilist[myid * c.npf + idj] : 0;
-            }
-#pragma unroll
-            for (int gj = 0; gj < G; gj++)
-                if (on && gj > slot && gj < n) pt[slot * G + gj] = row[gj];
-        }
-        wave_sync();
-
-        R pfrhs = R(0);
-        bool act = false;
-        bool unsure = false;
-        if (n > 1) {
-            // capillary driving force (src/cipher.c:474-497), replayed per slot
-            R cap = R(0), capabs = R(0);
-            for (int gk = 0; gk < n; gk++) {
-                const R phk = gsh<G>(phi, gk), lapk = gsh<G>(plap, gk);
-                const int idk = gsh<G>(myid, gk);
-                for (int gj = gk + 1; gj < n; gj++) {
-                    const R phj = gsh<G>(phi, gj), lapj = gsh<G>(plap, gj);
-                    const int idj = gsh<G>(myid, gj);
-                    const int it = pt[gk * G + gj];
-                    const R e = pair_e<R>(c, itab, it, T);
-                    const R w = (R)itab[3 * it + 1];
-                    R lapaj = R(0), lapak = R(0);
-                    if constexpr (MIXED) { lapaj = gsh<G>(lapabs, gj); lapak = gsh<G>(lapabs, gk); }
-                    if (slot == gk) {
-                        cap -= e * (phj / w + lapj * w / (R)CPI / (R)CPI);
-                        if constexpr (MIXED) capabs += fabsf(e) * (fabsf(phj / w) + lapaj * fabsf(w) / (R)CPI / (R)CPI);
-                    }
-                    if (slot == gj) {
-                        cap -= e * (phk / w + lapk * w / (R)CPI / (R)CPI);
-                        if constexpr (MIXED) capabs += fabsf(e) * (fabsf(phk / w) + lapak * fabsf(w) / (R)CPI / (R)CPI);
-                    }
-                    for (int gi = gj + 1; gi < n; gi++) {
-                        const R phi_i = gsh<G>(phi, gi);
-                        const int idi = gsh<G>(myid, gi);
-                        const int it2 = pt[gk * G + gi];
-                        const int it3 = pt[gj * G + gi];
-                        R tj = R(0);
-                        const R r1 = e / w;
-                        tj = tj > r1 ? tj : r1;
-                        const R r2 = pair_e<R>(c, itab, it2, T) / (R)itab[3 * it2 + 1];
-                        tj = tj > r2 ? tj : r2;
-                        const R r3 = pair_e<R>(c, itab, it3, T) / (R)itab[3 * it3 + 1];
-                        tj = tj > r3 ? tj : r3;
-                        tj *= (R)c.junctionpenalty;
-                        R t3 = R(0);
-                        if (slot == gk) t3 = tj * phj * phi_i;
-                        if (slot == gj) t3 = tj * phk * phi_i;
-                        if (slot == gi) t3 = tj * phk * phj;
-                        if (slot == gk || slot == gj || slot == gi) {
-                            cap -= t3;
-                            if constexpr (MIXED) capabs += fabsf(t3);
-                        }
-                    }
-                }
-                if (slot == gk) {
-                    cap *= R(8);
-                    cap += lapk;
-                    if constexpr (MIXED) capabs = R(8) * capabs + lapabs;
-                }
-            }
-            // chemical driving force (src/cipher.c:499-522; Chemenergy, src/material.c:480-490), FP64
-            R chs = R(0);
-            if (on) {
-                double ch = d_chemenergy<NC, double>(c, m, yd, Td);
-                ch /= m.molarvolume;
-#pragma unroll
-                for (int s = 0; s < 2; s++) {
-                    if (s >= ns) break;
-#pragma unroll
-                    for (int k = 0; k < ND; k++) ch -= x[c.dpo + k] * m.stoichiometry[s] * yd[s * NC + k];
-                }
-                chs = (R)ch;
-            }
-            if constexpr (NC > 1) {
-                if (c.has_potential) {
-                    R wdp[NDV];
-#pragma unroll
-                    for (int k = 0; k < ND; k++) wdp[k] = R(0);
-                    for (int g = 0; g < n; g++) {
-                        const int nsg = gsh<G>(ns, g);
-#pragma unroll
-                        for (int s = 0; s < 2; s++) {
-                            const R st = (s < ns) ? (R)m.stoichiometry[s] : R(0);
-#pragma unroll
-                            for (int k = 0; k < ND; k++) {
-                                const R v = gsh<G>(h * st * y[s * NC + k], g);
-                                if (s < nsg) wdp[k] += v;
-                            }
-                        }
-                    }
-                    for (int gk = 0; gk < n; gk++) {
-                        const int idk = gsh<G>(myid, gk);
-                        const R hk = gsh<G>(h, gk);
-                        for (int gj = gk + 1; gj < n; gj++) {
-                            const int idj = gsh<G>(myid, gj);
-                            const R hj = gsh<G>(h, gj);
-                            const cipher_interface &f = c.itf[c.ilist[idk * c.npf + idj]];
-                            if (f.has_potential) {
-                                R rv = R(0);
-#pragma unroll
-                                for (int k = 0; k < ND; k++) rv += (R)f.potential[k] * wdp[k];
-                                if (slot == gk) chs += rv * hj;
-                                if (slot == gj) chs += rv * hk;
-                            }
-                        }
-                    }
-                }
-            }
-            R chsabs = R(0);
-            if constexpr (MIXED) {
-                // magnitude of the inputs of the interpolant-derivative product
-                R mx = R(0);
-                for (int g = 0; g < n; g++) mx = fmaxf(mx, fabsf(gsh<G>(chs, g)));
-                chsabs = R(12) * mx;
-            }
-            chs = interp_deriv<G, R>(c, n, phi, hh, chs);
-            // unconstrained rate and active flags (src/cipher.c:524-539)
-            R ru = R(0), ruabs = R(0);
-            R nact = R(0);
-            for (int gk = 0; gk < n; gk++) {
-                const R capk = gsh<G>(cap, gk), chk = gsh<G>(chs, gk), hk = gsh<G>(h, gk);
-                const int idk = gsh<G>(myid, gk);
-                R capak = R(0);
-                if constexpr (MIXED) capak = gsh<G>(capabs, gk);
-                for (int gj = gk + 1; gj < n; gj++) {
-                    const R capj = gsh<G>(cap, gj), chj = gsh<G>(chs, gj), hj = gsh<G>(h, gj);
-                    const int idj = gsh<G>(myid, gj);
-                    const R mkj = pair_m<R>(c, itab, pt[gk * G + gj], T);
-                    const R rv = mkj * ((capk - capj) - (chk - chj) * R(8) * sqrt(hk * hj) / (R)CPI);
-                    if (slot == gk) ru += rv;
-                    if (slot == gj) ru -= rv;
-                    if constexpr (MIXED) {
-                        const R capaj = gsh<G>(capabs, gj);
-                        if (slot == gk || slot == gj)
-                            ruabs += fabsf(mkj) * (fabsf(capk) + fabsf(capj) + capak + capaj
-                                                   + (fabsf(chk) + fabsf(chj) + R(2) * chsabs) * R(8) * sqrtf(hk * hj) / (R)CPI);
-                    }
-                }
-                if (slot == gk) {
-                    act = (phid > CIPHER_TOL && phid < 1.0 - CIPHER_TOL) || (phid < CIPHER_TOL && ru > R(0))
-                          || (phid > 1.0 - CIPHER_TOL && ru < R(0));
-                    // the decision reads the sign of ru: is it inside the FP32 band?
-                    if constexpr (MIXED)
-                        unsure = !(phid > CIPHER_TOL && phid < 1.0 - CIPHER_TOL) && fabsf(ru) <= 1.0e-4f * ruabs;
-                }
-            }
-            for (int g = 0; g < n; g++) nact += gsh<G>((int)act, g) ? R(1) : R(0);
-            // constrained rate (src/cipher.c:541-555)
-            for (int gk = 0; gk < n; gk++) {
-                const int ak = gsh<G>((int)act, gk);
-                if (!ak) continue;
-                const R capk = gsh<G>(cap, gk), chk = gsh<G>(chs, gk), hk = gsh<G>(h, gk);
-                const int idk = gsh<G>(myid, gk);
-                for (int gj = gk + 1; gj < n; gj++) {
-                    const int aj = gsh<G>((int)act, gj);
-                    const R capj = gsh<G>(cap, gj), chj = gsh<G>(chs, gj), hj = gsh<G>(h, gj);
-                    const int idj = gsh<G>(myid, gj);
-                    if (!aj) continue;
-                    const R mkj = pair_m<R>(c, itab, pt[gk * G + gj], T);
-                    const R rv = mkj * ((capk - capj) - (chk - chj) * R(8) * sqrt(hk * hj) / (R)CPI);
-                    if (slot == gk) pfrhs += rv;
-                    if (slot == gj) pfrhs -= rv;
-                }
-                if (slot == gk) pfrhs /= nact;
-            }
-        }
-        const int gbase = (threadIdx.x & 63) & ~(G - 1);
-        const unsigned long long gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << gbase);
-        if constexpr (MIXED && !LIST) {
-            if (__ballot(unsure) & gmask) {
-                // leave this cell to the FP64 pass
-                if (slot == 0) {
-                    const int k = atomicAdd(a.list_count, 1);
-                    a.list[k] = (int32_t)cell;
-                }
-            }
-        }
-        if (a.active) {
-            const unsigned long long bal = __ballot(act);
-            if (slot == 0) a.active[cell] = (uint32_t)((bal & gmask) >> gbase);
-        }
-
-        // temperature rate (src/cipher.c:558-568)
-        R tmrhs = tlap.s;
-        if (c.has_theat || c.has_trate) {
-            R cpv = R(0);
-            const R cpt = (on && c.ts[m.specific_heat].n) ? h * dtseries<R>(&c.ts[m.specific_heat], T) : R(0);
-            const R lht = (on && c.ts[m.latent_heat].n) ? pfrhs * dtseries<R>(&c.ts[m.latent_heat], T) : R(0);
-            for (int g = 0; g < n; g++) {
-                const int idg = gsh<G>(myid, g);
-                const cipher_material &mg = c.mat[c.phase_material[idg]];
-                const R v1 = gsh<G>(cpt, g), v2 = gsh<G>(lht, g);
-                if (c.ts[mg.specific_heat].n) cpv += v1;
-                if (c.ts[mg.latent_heat].n) tmrhs += v2;
-            }
-            if (cpv != R(0)) tmrhs /= cpv;
-            if (c.has_trate) tmrhs += (R)c.trate;
-        }
-
-        double dprhs[NDV];
-#pragma unroll
-        for (int k = 0; k < NDV; k++) dprhs[k] = 0.0;
-        double cprd[2 * NDV];
-#pragma unroll
-        for (int k = 0; k < 2 * NDV; k++) cprd[k] = 0.0;
-        if constexpr (NC > 1) {
-            // explicit potential rate (src/cipher.c:572-581), FP64
-            if (on) {
-                double spw[2 * NDV];
-                d_sitepot_explicit<NC, double>(c, m, spw, yd, Td);
-#pragma unroll
-                for (int s = 0; s < 2; s++) {
-                    if (s >= ns) break;
-#pragma unroll
-                    for (int k = 0; k < ND; k++)
-                        cprd[s * ND + k] = m.chempot_ex_kineticcoeff * (spw[s * ND + k] - x[c.exo + slot * c.SP + s * ND + k]);
-                }
-            }
-            // composition rate (src/cipher.c:584-662), FP64 in both modes
-            const double hq = (double)h, hhq = (double)hh, pfq = (double)pfrhs, tmq = (double)tmrhs;
-            double wcp[NDV];
-#pragma unroll
-            for (int k = 0; k < ND; k++) wcp[k] = 0.0;
-            if (c.has_potential) {
-                for (int gk = 0; gk < n; gk++) {
-                    const double hk = gsh<G>(hq, gk);
-                    const int idk = gsh<G>(myid, gk);
-                    for (int gj = gk + 1; gj < n; gj++) {
-                        const double hj = gsh<G>(hq, gj);
-                        const int idj = gsh<G>(myid, gj);
-                        const cipher_interface &f = c.itf[c.ilist[idk * c.npf + idj]];
-                        if (f.has_potential) {
-#pragma unroll
-                            for (int k = 0; k < ND; k++) {
-                                if (slot == gk) wcp[k] += f.potential[k] * hj;
-                                if (slot == gj) wcp[k] += f.potential[k] * hk;
-                            }
-                        }
-                    }
-                }
-            }
-            double tpot[2 * ND * ND], tT[2 * ND];
-            double exs = 0.0;
-#pragma unroll
-            for (int k = 0; k < 2 * ND * ND; k++) tpot[k] = 0.0;
-#pragma unroll
-            for (int k = 0; k < 2 * ND; k++) tT[k] = 0.0;
-            if (on) {
-                d_sitefrac_tangent<NC, double>(c, m, tpot, tT, exs, yd, spimd, Td);
-            }
-            double dcdm[ND * ND];
-#pragma unroll
-            for (int q = 0; q < ND * ND; q++) dcdm[q] = 0.0;
-            for (int g = 0; g < n; g++) {
-                const int nsg = gsh<G>(ns, g);
-#pragma unroll
-                for (int s = 0; s < 2; s++) {
-                    const double st = (s < ns) ? m.stoichiometry[s] : 0.0;
-#pragma unroll
-                    for (int q = 0; q < ND * ND; q++) {
-                        const double v = gsh<G>(hq * st * st * tpot[s * ND * ND + q], g);
-                        if (s < nsg) dcdm[q] += v;
-                    }
-                }
-            }
-            {
-                double tmp[ND];
-#pragma unroll
-                for (int k = 0; k < ND; k++) tmp[k] = wcp[k];
-                dmatvec<ND, double>(wcp, dcdm, tmp);
-            }
-            double wdp[ND];
-#pragma unroll
-            for (int k = 0; k < ND; k++) wdp[k] = 0.0;
-            if (n > 1) {
-#pragma unroll
-                for (int q = 0; q < ND; q++) {
-                    double wpf = 0.0;
-#pragma unroll
-                    for (int s = 0; s < 2; s++)
-                        if (s < ns) wpf += m.stoichiometry[s] * yd[s * NC + q];
-                    wpf -= wcp[q];
-                    if (!on) wpf = 0.0;
-                    wpf = interp_deriv<G, double>(c, n, phid, hhq, wpf);
-                    for (int g = 0; g < n; g++) wdp[q] -= gsh<G>(pfq * wpf, g);
-                }
-            }
-            // Chemsource (src/material.c:321-351), called with the interpolants
-            double csrc[ND];
-#pragma unroll
-            for (int k = 0; k < ND; k++) csrc[k] = 0.0;
-            if (c.has_sources) {
-                for (int gk = 0; gk < n; gk++) {
-                    const int idk = gsh<G>(myid, gk);
-                    const double hk = gsh<G>(hq, gk);
-                    const cipher_material &mk = c.mat[c.phase_material[idk]];
-                    for (int s = 0; s < mk.nsources; s++) {
-                        const cipher_source &src = c.src[mk.source + s];
-                        if (src.model != CIPHER_SOURCE_SINK) continue;
-#pragma unroll
-                        for (int q = 0; q < ND; q++)
-                            csrc[q] += hk * hk * (src.rate[q] * (src.ceq[q] - cmS[q]));
-                    }
-                    for (int gj = gk + 1; gj < n; gj++) {
-                        const int idj = gsh<G>(myid, gj);
-                        const double hj = gsh<G>(hq, gj);
-                        const cipher_interface &f = c.itf[c.ilist[idk * c.npf + idj]];
-                        for (int s = 0; s < f.nsources; s++) {
-                            const cipher_source &src = c.src[f.source + s];
-                            if (src.model != CIPHER_SOURCE_SINK) continue;
-#pragma unroll
-                            for (int q = 0; q < ND; q++)
-                                csrc[q] += 2.0 * hk * hj * (src.rate[q] * (src.ceq[q] - cmS[q]));
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < ND; q++) wdp[q] += ((double)dlap[q].s + csrc[q]);
-            // sum_g sum_s h nu (T_ex . EXdot + T_T * Tdot)
-            for (int g = 0; g < n; g++) {
-                const int nsg = gsh<G>(ns, g);
-#pragma unroll
-                for (int s = 0; s < 2; s++) {
-                    double tv[ND];
-                    double tex[ND * ND];
-#pragma unroll
-                    for (int q = 0; q < ND * ND; q++) tex[q] = exs == 0.0 ? 0.0 : -tpot[s * ND * ND + q];
-                    double cps[ND];
-#pragma unroll
-                    for (int q = 0; q < ND; q++) cps[q] = cprd[s * ND + q];
-                    dmatvec<ND, double>(tv, tex, cps);
-                    const double st = (s < ns) ? m.stoichiometry[s] : 0.0;
-#pragma unroll
-                    for (int q = 0; q < ND; q++) {
-                        const double v = gsh<G>(hq * st * (tv[q] + tT[s * ND + q] * tmq), g);
-                        if (s < nsg) wdp[q] -= v;
-                    }
-                }
-            }
-            double dmdc[ND * ND];
-            dinvert<ND, double>(dmdc, dcdm);
-            dmatvec<ND, double>(dprhs, dmdc, wdp);
-        }
-
-        // ---- assemble F in LDS ------------------------------------------------
-        if (on) fr[c.pfo + slot] = (double)pfrhs;
-        if (slot == 0) {
-            fr[c.tmo] = (double)tmrhs;
-#pragma unroll
-            for (int q = 0; q < ND; q++) fr[c.dpo + q] = dprhs[q];
-        }
-        if constexpr (NC > 1) {
-            if (on) {
-#pragma unroll
-                for (int s = 0; s < 2; s++) {
-                    if (s >= ns) break;
-#pragma unroll
-                    for (int q = 0; q < ND; q++) fr[c.exo + slot * c.SP + s * ND + q] = cprd[s * ND + q];
-                }
-            }
-        }
-    }
-    __syncthreads();
-    if (LIST) {
-        if (lc < nblk)
-            for (int k = slot; k < D; k += G) a.F[cell * D + k] = ldsF[lc * D + k];
-    } else {
-        stage_out(a.F, ldsF, c0, nblk, D);
-    }
-    }  // block loop
-}
-
-__global__ void k_zero_i32(int32_t *p) { *p = 0; }
-
-// ---------------------------------------------------------------------------
-template <int G, int NC>
-static cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st) {
-    constexpr int CPB = 256 / G;
-    const long long nb = (L.ncells + CPB - 1) / CPB;
-    if (nb == 0) return CIPHER_OK;
-    const DevCfg &c = *L.cfg;
-    const bool need_prep = (NC > 1) || c.has_tcond;
-    if (need_prep) {
-        const size_t sh = (size_t)CPB * c.D * sizeof(double);
-        hipLaunchKernelGGL((k_prep<G, NC>), dim3((unsigned)nb), dim3(256), sh, st, L.X, (double *)L.cm, L.ncells, c);
-    }
-    RhsArgs a;
-    a.X = L.X; a.F = L.F; a.active = L.active; a.cellgeom = L.cellgeom;
-    a.cell_off = L.cell_off; a.adj = L.adj; a.bcell_off = L.bcell_off; a.bface_id = L.bface_id;
-    a.cm = (const double *)L.cm; a.ncells = L.ncells;
-    a.list = L.list; a.list_count = L.list_count;
-    const size_t sh = 2 * (size_t)CPB * c.D * sizeof(double) + 3 * (size_t)c.nf * sizeof(double)
-                      + (size_t)CPB * G * G * sizeof(int);
-    if (precision == CIPHER_FP32_MIXED) {
-        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, L.list_count);
-        hipLaunchKernelGGL((k_rhs<G, NC, float, false>), dim3((unsigned)nb), dim3(256), sh, st, a, c);
-        // FP64 recomputation of the cells whose flags sat in the FP32 band
-        const long long rb = nb < 512 ? nb : 512;
-        hipLaunchKernelGGL((k_rhs<G, NC, double, true>), dim3((unsigned)rb), dim3(256), sh, st, a, c);
-    } else {
-        hipLaunchKernelGGL((k_rhs<G, NC, double, false>), dim3((unsigned)nb), dim3(256), sh, st, a, c);
-    }
-    return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
-}
-
-template <int NC>
-static cipher_status launch_NC(const RhsLaunch &L, int precision, hipStream_t st) {
-    const int P = L.cfg->P;
-    if (P <= 4) return launch_G<4, NC>(L, precision, st);
-    if (P <= 8) return launch_G<8, NC>(L, precision, st);
-    if (P <= 16) return launch_G<16, NC>(L, precision, st);
-    return CIPHER_ERR_UNSUPPORTED;
-}
+cipher_status launch_NC(const RhsLaunch &L, int precision, hipStream_t st);
 
 cipher_status launch_rhs(const RhsLaunch &L, int precision, hipStream_t st) {
     switch (L.cfg->ncp) {
@@ -936,6 +17,19 @@ cipher_status launch_rhs(const RhsLaunch &L, int precision, hipStream_t st) {
     case 5: return launch_NC<5>(L, precision, st);
     default: return CIPHER_ERR_UNSUPPORTED;
     }
+}
+
+// fold every T-series at the temperature of the first cell (deterministic:
+// the same code as the direct evaluation)
+__global__ void k_tsfold(const double *X, long long ncells, DevCfg c, double *tsv) {
+    const double T = ncells > 0 ? X[c.tmo] : 0.0;
+    for (int i = threadIdx.x; i < c.ntseries; i += blockDim.x) tsv[i] = dtseries<double>(&c.ts[i], T);
+    if (threadIdx.x == 0) tsv[c.ntseries] = T;
+}
+
+cipher_status launch_tsfold(const double *X, long long ncells, const DevCfg &c, double *tsv, hipStream_t st) {
+    hipLaunchKernelGGL(k_tsfold, dim3(1), dim3(256), 0, st, X, ncells, c, tsv);
+    return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
 }
 
 size_t prep_record_bytes(int ncp) { return (size_t)(2 * ncp + 1) * sizeof(double); }

@@ -1,0 +1,6 @@
+// rhs_nc3.hip - right-hand-side kernels instantiated for 3 component(s).
+#include "rhs_impl.h"
+
+namespace cipher {
+template cipher_status launch_NC<3>(const RhsLaunch &L, int precision, hipStream_t st);
+}  // namespace cipher
