@@ -110,7 +110,7 @@ def grain_growth_yaml(res=(128, 128), ngrains=20, periodic=True, capacity=8, t_e
         centres = lo + (np.arange(ntypes) + 0.5) * (hi - lo) / ntypes
         interfaces = [f"gb{t}" for t in range(ntypes)]
         inter = "interface:\n" + "".join(
-            f"  gb{t}:\n    width : {width!r}\n    energy :\n      e0: {centres[t]!r}\n"
+            f"  gb{t}:\n    width : {float(width)!r}\n    energy :\n      e0: {float(centres[t])!r}\n"
             f"    mobility :\n      m0: 1.0\n" for t in range(ntypes)) + "\n"
         imap = _interface_rle(ngrains, lambda i, j: types[i, j] + 1)
     text = _header(list(res), [float(r) for r in res], ngrains, ["matrix"], interfaces, ["ti"])
@@ -151,7 +151,7 @@ def ternary_ppt_yaml(res=(128, 128, 128), ngrains=100, nppt=40, radius=3, capaci
     def mob_block(ind):
         s = " " * ind + "mobilityc :\n"
         for c, nm in enumerate(comps):
-            s += " " * (ind + 2) + f"{nm} :\n" + " " * (ind + 4) + f"mobility0 : {M[c] * RT!r}\n"
+            s += " " * (ind + 2) + f"{nm} :\n" + " " * (ind + 4) + f"mobility0 : {float(M[c] * RT)!r}\n"
         return s
 
     lmax = int(round(math.log2(res[0])))
@@ -160,16 +160,16 @@ def ternary_ppt_yaml(res=(128, 128, 128), ngrains=100, nppt=40, radius=3, capaci
                    ["grainboundary", "pptboundary", "pptppt"], comps)
     text += _solution([t_end], [1] * dim, lmax, minlevel, lmax, capacity, 1e-12, t_end, per)
     text += "material:\n  matrix:\n    chemicalenergy : calphaddis\n    molarvolume : 1e-5\n"
-    text += f"    temperature0: {temperature!r}\n    chempot_ex_kineticcoeff : 1.0\n"
+    text += f"    temperature0: {float(temperature)!r}\n    chempot_ex_kineticcoeff : 1.0\n"
     text += "    c0 : [0.90, 0.07, 0.03]\n" + mob_block(4)
     text += "    binary_enthalpy :\n"
     pairs = [(0, 1), (0, 2), (1, 2)]
     for k, (i, j) in enumerate(pairs):
         text += (f"      {comps[i]} :\n        {comps[j]} :\n          nrk : 2\n"
-                 f"          rk_0 :\n            t_coefficient : {L0[k]!r}\n            t_exponent : 0\n"
-                 f"          rk_1 :\n            t_coefficient : {L1[k]!r}\n            t_exponent : 0\n")
+                 f"          rk_0 :\n            t_coefficient : {float(L0[k])!r}\n            t_exponent : 0\n"
+                 f"          rk_1 :\n            t_coefficient : {float(L1[k])!r}\n            t_exponent : 0\n")
     text += "  precipitate:\n    chemicalenergy : calphad2sl\n    molarvolume : 1e-5\n"
-    text += f"    temperature0: {temperature!r}\n    chempot_ex_kineticcoeff : 1.0\n"
+    text += f"    temperature0: {float(temperature)!r}\n    chempot_ex_kineticcoeff : 1.0\n"
     text += "    stochiometry : [0.75, 0.25]\n"
     text += "    c0_p : [0.70, 0.29, 0.01]\n    c0_q : [0.01, 0.01, 0.98]\n" + mob_block(4)
     text += "    unary_enthalpy :\n"
@@ -179,12 +179,12 @@ def ternary_ppt_yaml(res=(128, 128, 128), ngrains=100, nppt=40, radius=3, capaci
                 g = G_em[cp]
             else:
                 g = penalty
-            text += f"      ({comps[cp]}:{comps[cq]}) :\n        t_coefficient : {g!r}\n        t_exponent : 0\n"
+            text += f"      ({comps[cp]}:{comps[cq]}) :\n        t_coefficient : {float(g)!r}\n        t_exponent : 0\n"
     text += "\n"
     width = 4.0 * dx
     text += "interface:\n"
     for nm, e0 in (("grainboundary", 0.5), ("pptboundary", 0.3), ("pptppt", 0.5)):
-        text += (f"  {nm}:\n    width : {width!r}\n    energy :\n      e0: {e0}\n"
+        text += (f"  {nm}:\n    width : {float(width)!r}\n    energy :\n      e0: {e0}\n"
                  f"    mobility :\n      m0: 1.0e-13\n")
     text += "\n"
 

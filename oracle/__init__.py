@@ -94,3 +94,126 @@ def wrms_norm(U, Y, atol, rtol):
     U = np.ascontiguousarray(U, dtype=np.float64)
     Y = np.ascontiguousarray(Y, dtype=np.float64)
     return lib().orc_wrms_norm(_p(U), _p(Y), U.size, atol, rtol)
+
+
+# ---- per-function wrappers (for the pinning tests) -------------------------
+def fastpow(b, e):
+    return lib().orc_fastpow(float(b), int(e))
+
+
+def tseries(coeff, exps, logc=0.0):
+    t = abi.TSeries()
+    t.n = len(coeff)
+    for i, (cf, ex) in enumerate(zip(coeff, exps)):
+        t.coeff[i] = cf
+        t.exp[i] = ex
+    t.logcoeff = logc
+    return t
+
+
+def sum_tseries(T, t):
+    return lib().orc_sum_tseries(float(T), C.byref(t))
+
+
+def sum_tseries_derivative(T, t):
+    return lib().orc_sum_tseries_derivative(float(T), C.byref(t))
+
+
+def _u16(lst):
+    a = np.zeros(64, dtype=np.uint16)
+    a[0] = len(lst)
+    a[1:1 + len(lst)] = lst
+    return a
+
+
+def set_intersection(A, B):
+    a, b = _u16(A), _u16(B)
+    out, ia, ib = np.zeros(64, np.uint16), np.zeros(64, np.uint16), np.zeros(64, np.uint16)
+    lib().orc_set_intersection(_p(out), _p(ia), _p(ib), _p(a), _p(b))
+    k = int(out[0])
+    return list(out[1:1 + k]), list(ia[:k]), list(ib[:k])
+
+
+def set_union(A, B):
+    a, b = _u16(A), _u16(B)
+    out, ia, ib = np.zeros(64, np.uint16), np.zeros(64, np.uint16), np.zeros(64, np.uint16)
+    lib().orc_set_union(_p(out), _p(ia), _p(ib), _p(a), _p(b))
+    k = int(out[0])
+    return list(out[1:1 + k]), list(ia[:len(A)]), list(ib[:len(B)])
+
+
+def invert(M):
+    M = np.ascontiguousarray(M, dtype=np.float64).copy()
+    n = M.shape[0]
+    out = np.zeros_like(M)
+    lib().orc_invert_matrix(_p(out), _p(M), n)
+    return out
+
+
+def interpolant(kind, phi):
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    h = np.zeros_like(phi)
+    lib().orc_interpolant(kind, _p(h), _p(phi), phi.size)
+    return h
+
+
+def interpolant_derivative(kind, vec, phi):
+    v = np.ascontiguousarray(vec, dtype=np.float64).copy()
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    lib().orc_interpolant_derivative(kind, _p(v), _p(phi), phi.size)
+    return v
+
+
+def simplex_projection(v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.zeros_like(v)
+    lib().orc_simplex_projection(_p(out), _p(v), v.size)
+    return out
+
+
+def _mat_call(name, cfg, mat, nout, arg, T):
+    out = np.zeros(nout, dtype=np.float64)
+    a = np.ascontiguousarray(arg, dtype=np.float64)
+    getattr(lib(), name)(C.byref(cfg.struct), mat, _p(out), _p(a), float(T))
+    return out
+
+
+def sitefrac(cfg, mat, spim, T):
+    return _mat_call("orc_sitefrac", cfg, mat, 2 * cfg.ncp, spim, T)
+
+
+def sitepot_explicit(cfg, mat, y, T):
+    return _mat_call("orc_sitepot_explicit", cfg, mat, 2 * (cfg.ncp - 1), y, T)
+
+
+def sitepot_implicit(cfg, mat, y, T):
+    return _mat_call("orc_sitepot_implicit", cfg, mat, 2 * (cfg.ncp - 1), y, T)
+
+
+def mobility_component(cfg, mat, y, T):
+    return _mat_call("orc_mobility_component", cfg, mat, cfg.ncp, y, T)
+
+
+def chemenergy_phase(cfg, mat, y, T):
+    a = np.ascontiguousarray(y, dtype=np.float64)
+    return lib().orc_chemenergy_phase(C.byref(cfg.struct), mat, _p(a), float(T))
+
+
+def sitefrac_tangent(cfg, mat, y, spim, T):
+    nd = cfg.ncp - 1
+    tp = np.zeros(2 * nd * nd)
+    tx = np.zeros(2 * nd * nd)
+    tt = np.zeros(2 * nd)
+    a = np.ascontiguousarray(y, dtype=np.float64)
+    s = np.ascontiguousarray(spim, dtype=np.float64)
+    lib().orc_sitefrac_tangent(C.byref(cfg.struct), mat, _p(tp), _p(tx), _p(tt), _p(a), _p(s), float(T))
+    return tp, tx, tt
+
+
+def mobility_volumeref(cfg, mob, comp):
+    nd = cfg.ncp - 1
+    out = np.zeros(max(nd * nd, 1))
+    m = np.ascontiguousarray(mob, dtype=np.float64)
+    c = np.ascontiguousarray(comp, dtype=np.float64)
+    lib().orc_mobility_volumeref(C.byref(cfg.struct), _p(out), _p(m), _p(c))
+    return out[:nd * nd].reshape(nd, nd)

@@ -40,7 +40,8 @@ def initial_state(wl):
 
 def evolve(wl, X, steps=10, frac=0.02, seed=0):
     """Explicit Euler + active-set update with the oracle; dt picked so the
-    largest phase-fraction change per step is `frac`."""
+    largest phase-fraction change per step is `frac` and no entry changes by
+    more than 5% of max(|x|, 1)."""
     L = wl.layout
     D = L["ndof"]
     P = L["pf_size"]
@@ -49,6 +50,10 @@ def evolve(wl, X, steps=10, frac=0.02, seed=0):
         Fv = F.reshape(-1, D)
         m = np.abs(Fv[:, L["pf_offset"]:L["pf_offset"] + P]).max()
         dt = frac / m if m > 0 else 1.0
+        # and no field changes by more than 5% of its size (stiff chemistry)
+        r = (np.abs(Fv) / np.maximum(np.abs(X.reshape(-1, D)), 1.0)).max()
+        if r > 0:
+            dt = min(dt, 0.05 / r)
         X = X + dt * F
         X, rc = oracle.poststep(wl.cfg, wl.mesh, X)
         assert rc == 0
@@ -121,3 +126,139 @@ def state_mask(X, L, cfg):
                 for c in range(nd):
                     mask[:, L["ex_offset"] + k * L["sp_size"] + s * nd + c] = ok & (nsites > s)
     return mask.ravel()
+
+
+def chem_test_yaml(ncomp=4, sources=True, thermal=False):
+    """A small 2D config with every constitutive model: calphaddis with RK
+    binaries (orders 0-2) and a ternary term, calphad2sl with unary/binary/
+    ternary tables on both sublattices, quadratic, none; interfaces with a
+    segregation potential and solute mobility factors; a sink source."""
+    comps = ["al", "cu", "mg", "zn", "si"][:ncomp]
+    rng = np.random.default_rng(7)
+
+    def ts(ind, coeffs, exps, tlnt=None):
+        s = " " * ind + "t_coefficient : [" + ", ".join(repr(float(c)) for c in coeffs) + "]\n"
+        s += " " * ind + "t_exponent : [" + ", ".join(str(e) for e in exps) + "]\n"
+        if tlnt is not None:
+            s += " " * ind + f"tlnt_coefficient : {tlnt!r}\n"
+        return s
+
+    def mob(ind):
+        s = " " * ind + "mobilityc :\n"
+        for c in comps:
+            s += " " * (ind + 2) + f"{c} :\n" + " " * (ind + 4) + "mobility0 : 1.0e-5\n"
+            s += " " * (ind + 4) + "unary_migration :\n"
+            for cj in comps:
+                s += " " * (ind + 6) + f"{cj} :\n" + ts(ind + 8, [-1.0e4 * rng.random(), -5.0], [0, 1])
+            s += " " * (ind + 4) + "binary_migration :\n"
+            s += " " * (ind + 6) + f"{comps[0]} :\n" + " " * (ind + 8) + f"{comps[1]} :\n"
+            s += " " * (ind + 10) + "nrk : 2\n"
+            s += " " * (ind + 10) + "rk_0 :\n" + ts(ind + 12, [3000.0], [0])
+            s += " " * (ind + 10) + "rk_1 :\n" + ts(ind + 12, [-1500.0, 2.0], [0, 1])
+        return s
+
+    n = len(comps)
+    txt = ("---\nheader:\n  grid : [8, 8]\n  size : [8.0, 8.0]\n  n_phases : 4\n"
+           "  materials : [dis, sl, quad, none]\n  interfaces : [ia, ib]\n"
+           f"  components : [{', '.join(comps)}]\n  outputs : [phaseid]\n"
+           + ("  boundaries : [bl, br]\n" if thermal else "") + "\n"
+           "solution_parameters:\n  time : [1.0]\n  initblocksize : [1, 1]\n  initrefine : 3\n"
+           "  maxnrefine : 3\n  minnrefine : 3\n  initcoarsen : 0\n  amrinterval : 100000\n"
+           "  interpolation: cubic\n  max_phase_neighbours : 4\n"
+           + ("  temperature_rate : [2.5]\n  petscoptions : -dm_p4est_brick_periodicity 0,1\n\n" if thermal
+              else "  petscoptions : -dm_p4est_brick_periodicity 1,1\n\n"))
+    txt += "material:\n  dis:\n    chemicalenergy : calphaddis\n    molarvolume : 1.0e-5\n"
+    txt += "    temperature0 : 650.0\n    chempot_ex_kineticcoeff : 2.0\n"
+    c0 = rng.dirichlet(np.ones(n) * 3)
+    txt += "    c0 : [" + ", ".join(repr(float(v)) for v in c0) + "]\n" + mob(4)
+    txt += "    ref_enthalpy :\n" + ts(6, [100.0, -1.0], [0, 1])
+    if thermal:
+        txt += ("    tconductivity :\n" + ts(6, [2.0, 1.0e-3], [0, 1]) + "    specific_heat :\n"
+                + ts(6, [3.0e6], [0]) + "    latent_heat :\n" + ts(6, [-1.0e8], [0]))
+    txt += "    unary_enthalpy :\n"
+    for c in comps:
+        txt += f"      {c} :\n" + ts(8, [-8000.0 * rng.random(), 130.0, -2.0e-3, 5.0e4], [0, 1, 2, -1], tlnt=-24.0)
+    txt += "    binary_enthalpy :\n"
+    for i in range(n):
+        txt += f"      {comps[i]} :\n"
+        for j in range(i + 1, n):
+            txt += f"        {comps[j]} :\n          nrk : 3\n"
+            for o in range(3):
+                txt += f"          rk_{o} :\n" + ts(12, [2.0e4 * (rng.random() - 0.5), 3.0], [0, 1])
+    if n >= 3:
+        txt += (f"    ternary_enthalpy :\n      {comps[0]} :\n        {comps[1]} :\n          {comps[2]} :\n"
+                "            nrk : 2\n            rk_0 :\n" + ts(14, [5000.0], [0]) + "              index : 1\n"
+                "            rk_1 :\n" + ts(14, [-3000.0], [0]) + "              index : 2\n")
+    txt += "  sl:\n    chemicalenergy : calphad2sl\n    molarvolume : 1.0e-5\n"
+    txt += "    temperature0 : 650.0\n    chempot_ex_kineticcoeff : 1.0\n"
+    cp, cq = rng.dirichlet(np.ones(n) * 3), rng.dirichlet(np.ones(n) * 3)
+    txt += "    c0_p : [" + ", ".join(repr(float(v)) for v in cp) + "]\n"
+    txt += "    c0_q : [" + ", ".join(repr(float(v)) for v in cq) + "]\n"
+    txt += "    stochiometry : [0.75, 0.25]\n" + mob(4)
+    if thermal:
+        txt += ("    tconductivity :\n" + ts(6, [3.0], [0]) + "    specific_heat :\n"
+                + ts(6, [2.5e6, 100.0], [0, 1]) + "    latent_heat :\n" + ts(6, [2.0e8], [0]))
+    txt += "    unary_enthalpy :\n"
+    for a in comps:
+        for b in comps:
+            txt += f"      ({a}:{b}) :\n" + ts(8, [-2.0e4 * rng.random(), 1.0], [0, 1], tlnt=-1.5)
+    txt += "    binary_enthalpy :\n"
+    txt += f"      ({comps[0]},{comps[1]}:{comps[0]}) :\n        nrk : 2\n        rk_0 :\n" + ts(10, [4000.0], [0])
+    txt += "        rk_1 :\n" + ts(10, [-700.0], [0])
+    txt += f"      ({comps[1]}:{comps[0]},{comps[2 % n]}) :\n        nrk : 2\n        rk_0 :\n" + ts(10, [-2500.0], [0])
+    txt += "        rk_1 :\n" + ts(10, [900.0], [0])
+    if n >= 3:
+        txt += f"    ternary_enthalpy :\n      ({comps[0]},{comps[1]},{comps[2]}:{comps[0]}) :\n        nrk : 1\n        rk_0 :\n" + ts(10, [1200.0], [0])
+        txt += f"      ({comps[1]}:{comps[0]},{comps[1]},{comps[2]}) :\n        nrk : 1\n        rk_0 :\n" + ts(10, [-800.0], [0])
+    txt += "  quad:\n    chemicalenergy : quadratic\n    molarvolume : 1.0e-5\n    temperature0 : 650.0\n"
+    txt += "    c0 : [" + ", ".join(repr(float(v)) for v in rng.dirichlet(np.ones(n) * 3)) + "]\n" + mob(4)
+    txt += "    ref_enthalpy :\n" + ts(6, [10.0], [0])
+    for key, scale in (("ceq", None), ("unary_enthalpy", 1000.0), ("binary_enthalpy", 5.0e4)):
+        txt += f"    {key} :\n"
+        vals = rng.dirichlet(np.ones(n) * 4) if scale is None else scale * (1 + rng.random(n))
+        for c, v in zip(comps, vals):
+            # binary coefficients constant in T: the temperature tangent
+            # (src/material.c:924-970) holds the inverted matrix fixed
+            tdep = key != "binary_enthalpy"
+            txt += f"      {c} :\n" + (ts(8, [0.9 * float(v), 0.1 * float(v) / 650.0], [0, 1]) if tdep
+                                      else ts(8, [float(v)], [0]))
+    if sources:
+        txt += "    sources :\n      activesources : [sk]\n      sk :\n        source : sink\n"
+        txt += "        rate : [" + ", ".join("1.0e3" for _ in comps) + "]\n"
+        txt += "        ceq : [" + ", ".join(repr(float(v)) for v in rng.dirichlet(np.ones(n))) + "]\n"
+    txt += "  none:\n    chemicalenergy : none\n    molarvolume : 1.0e-5\n    temperature0 : 650.0\n\n"
+    txt += ("interface:\n  ia:\n    width : 3.0\n    energy :\n      e0: 0.5\n    mobility :\n      m0: 1.0e-3\n"
+            "    potential : [" + ", ".join("-500.0" for _ in comps[:-1]) + "]\n"
+            "    mobilityc : [" + ", ".join("3000.0" for _ in comps) + "]\n"
+            "  ib:\n    width : 3.0\n    energy :\n      e0: 0.7\n      activation_energy :\n"
+            + ts(8, [-2000.0], [0]) + "    mobility :\n      m0: 2.0e-3\n      activation_energy :\n"
+            + ts(8, [-1000.0], [0]) + "\n")
+    if thermal:
+        # x low: chemical Neumann on two components, thermal Dirichlet;
+        # x high: chemical Dirichlet on one component, thermal Neumann
+        stars = lambda vals: "[" + ", ".join(vals) + "]"
+        v1 = ["1.0e3"] * (n - 1)
+        v1[1 % (n - 1)] = "'*'"
+        v2 = ["'*'"] * (n - 1)
+        v2[0] = "-1.0e7"
+        txt += ("boundary:\n  bl:\n    boundary_id : 1\n    chem :\n      type : neumann\n"
+                f"      value : {stars(v1)}\n    thermal :\n      type : dirichlet\n      value : [640.0]\n"
+                "  br:\n    boundary_id : 2\n    chem :\n      type : dirichlet\n"
+                f"      value : {stars(v2)}\n    thermal :\n      type : neumann\n      value : [-3.0]\n\n")
+    labels = np.arange(64).reshape(8, 8) // 16          # 4 stripes of phases 0..3
+    # phase 3 reuses 'dis': a cell whose phases all have no chemistry has a zero
+    # composition, and the harmonic face mean of two zero compositions is 0/0
+    # (src/cipher.c:280); 'none' stays in the material table for the
+    # per-function checks
+    txt += "mappings:\n  phase_material_mapping : |-\n    1\n    2\n    3\n    1\n"
+    from cipher_amd import config as cfgmod
+    txt += "  voxel_phase_mapping: |-\n" + "\n".join("    " + l for l in cfgmod.encode_rle(labels.T.ravel() + 1).split("\n")) + "\n"
+    txt += "  interface_mapping: |-\n    1\n    1\n    2\n    2\n    1\n    1\n    2\n    2\n    1\n    1\n    1\n    2\n    1\n    1\n    1\n    1\n"
+    return txt
+
+
+def chem(thermal=False, sources=True):
+    key = ("chem", thermal, sources)
+    if key not in _cache:
+        _cache[key] = W.Workload(chem_test_yaml(sources=sources, thermal=thermal), "chem")
+    return _cache[key]

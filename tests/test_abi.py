@@ -1,0 +1,105 @@
+"""The C ABI: header, ctypes mirror and the built library agree (no GPU).
+
+No compute entry point is called here; only the host-only layout query and
+the error path of context creation on a machine without a HIP device.
+"""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from cipher_amd import abi
+from tests import helpers as H
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cipher_amd.h")
+
+
+def _declared():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(cipher_[a-z_0-9]+)\(", txt, re.M)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(abi.LIB_PATH):
+        from cipher_amd.csrc import build
+        build.build()
+    return abi.load_library()
+
+
+def test_header_and_ctypes_declare_the_same_entry_points():
+    assert _declared() == sorted(abi.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", abi.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    syms = {l.split()[-1] for l in out.splitlines() if l.strip()}
+    for name in _declared():
+        assert name in syms, name
+        assert getattr(lib, name) is not None
+
+
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    names = ["cipher_tseries", "cipher_rk", "cipher_mobility", "cipher_source", "cipher_material",
+             "cipher_interface", "cipher_bc", "cipher_config", "cipher_layout", "cipher_mesh"]
+    mirrors = [abi.TSeries, abi.RK, abi.Mobility, abi.Source, abi.Material, abi.Interface, abi.BC,
+               abi.Config, abi.Layout, abi.Mesh]
+    fields = [("cipher_config", "junctionpenalty", abi.Config), ("cipher_config", "gradient_calculation", abi.Config),
+              ("cipher_material", "model", abi.Material), ("cipher_material", "source", abi.Material),
+              ("cipher_interface", "e_act", abi.Interface), ("cipher_bc", "boundary_id", abi.BC),
+              ("cipher_mesh", "ncells", abi.Mesh), ("cipher_tseries", "exp", abi.TSeries)]
+    src = tmp_path / "sz.c"
+    body = "".join(f'printf("%zu\\n", sizeof({n}));' for n in names)
+    body += "".join(f'printf("%zu\\n", offsetof({s}, {f}));' for s, f, _ in fields)
+    src.write_text(f'#include <stdio.h>\n#include <stddef.h>\n#include "{HEADER}"\nint main(void){{{body}return 0;}}\n')
+    exe = tmp_path / "sz"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-o", str(exe), str(src)], check=True)
+    vals = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [C.sizeof(m) for m in mirrors] + [getattr(t, f).offset for _, f, t in fields]
+    assert vals == want
+
+
+def test_constants_match_header():
+    txt = open(HEADER).read()
+    tol = re.search(r"#define CIPHER_TOL (\S+)", txt).group(1)
+    assert float(tol) == abi.TOL
+    assert f"#define CIPHER_MAXCP {abi.MAXCP}" in txt and f"#define CIPHER_MAXAP {abi.MAXAP}" in txt
+    assert f"#define CIPHER_R_GAS_CONST {abi.R_GAS_CONST}" in txt
+    for k, v in abi.STATUS.items():
+        assert f"{v} = {k}" in txt
+
+
+@pytest.mark.parametrize("name", ["gg2d", "ternary", "chem"])
+def test_layout_of_matches_host_layout(lib, name):
+    wl = getattr(H, name)()
+    out = abi.Layout()
+    assert lib.cipher_layout_of(C.byref(wl.cfg.struct), C.byref(out)) == 0
+    want = wl.layout
+    for k, _ in abi.Layout._fields_:
+        assert getattr(out, k) == want[k], k
+
+
+def test_layout_of_rejects_null(lib):
+    assert lib.cipher_layout_of(None, None) == 1
+    assert lib.cipher_last_error()
+
+
+def test_context_creation_fails_cleanly_without_a_device(lib):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is visible")
+    ctx = C.c_void_p()
+    wl = H.gg2d()
+    rc = lib.cipher_ctx_create(C.byref(wl.cfg.struct), abi.FP64, C.byref(ctx))
+    assert rc != 0 and not ctx.value
+    assert lib.cipher_last_error()
+
+
+def test_runtime_refuses_to_run_without_the_library(monkeypatch, tmp_path):
+    monkeypatch.setattr(abi, "_lib", None)
+    with pytest.raises(ImportError):
+        abi.load_library(str(tmp_path / "missing.so"))

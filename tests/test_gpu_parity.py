@@ -39,6 +39,12 @@ WORKLOADS = {
     "gg2d": lambda: H.gg2d(),
     "gg3d_amr": lambda: H.gg3d_amr(),
     "ternary": lambda: H.ternary(),
+    # four components, every constitutive model, segregation potential,
+    # solute mobility factors, sink source
+    "chem4": lambda: H.chem(),
+    # the same with conduction, latent and specific heat, a temperature rate
+    # and chemical/thermal Neumann and Dirichlet boundaries
+    "chem4_thermal": lambda: H.chem(thermal=True),
 }
 
 
@@ -105,7 +111,8 @@ def test_poststep_matches_oracle(name):
     F = oracle.rhs(wl.cfg, wl.mesh, X)
     Fv = F.reshape(-1, D)
     m = np.abs(Fv[:, L["pf_offset"]:L["pf_offset"] + L["pf_size"]]).max()
-    Xs = X + (0.05 / m) * F           # a stage value with phases leaving the simplex
+    r = (np.abs(Fv) / np.maximum(np.abs(X.reshape(-1, D)), 1.0)).max()
+    Xs = X + min(0.05 / m, 0.05 / r) * F     # a stage value with phases leaving the simplex
     print(f"[post] {name} cells={wl.ncells}", flush=True)
     Xo, rc = oracle.poststep(wl.cfg, wl.mesh, Xs)
     assert rc == 0
