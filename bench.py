@@ -7,8 +7,10 @@ interfaces, 500 Voronoi grains, interface energies U[0.8,1.2] per pair
 (quantised to 255 types), capacity 8, FP64 drop-in; the FP32-mixed path is
 timed beside it on the same state and its error against FP64 is reported.
 
-The state is the SetUpProblem initial condition advanced by a few explicit
-steps on the GPU (diffuse interfaces).  The CPU baseline is the C oracle
+The state is the SetUpProblem initial condition whose phase fields are
+advanced by a few explicit steps on the GPU (diffuse interfaces; chemical
+fields held at their initial values, the stiff chemistry would need the
+adaptive integrator).  The CPU baseline is the C oracle
 (single thread) on the same state, timed for a bounded number of calls.
 
 Prints one JSON line.  Run: python bench.py [--steps K] [--warmup W]
@@ -53,6 +55,30 @@ def algorithmic_bytes(ncells, nfaces, ndof):
     return ncells * (16 * ndof + 8) + 8 * nfaces
 
 
+def copy_probe(nbytes, reps=5):
+    """A plain device-to-device copy of known size: what FETCH_SIZE and
+    WRITE_SIZE read for it calibrates the counters against bytes moved."""
+    import torch
+    n = nbytes // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 8, "reps": reps}))
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the FP64 RHS kernel from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"k_rhs_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch_fp64\s*=\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,7 +88,12 @@ def main():
     ap.add_argument("--evolve", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the path shards by Morton ranges with a one-cell halo; see the design notes)", file=sys.stderr)
@@ -91,10 +122,15 @@ def main():
         X = torch.as_tensor(Xh).to(ctx.device)
     F = torch.empty_like(X)
     pf = slice(L["pf_offset"], L["pf_offset"] + P)
+    chem = slice(L["dp_offset"], L["tm_offset"])
     good = X.clone()
     evolved = 0
     for _ in range(args.evolve):
         ctx.rhs(X, F)
+        # phase fields only: an explicit step sized for the phase fields is far
+        # beyond the stability limit of the stiff chemical-potential dynamics,
+        # so mu and the explicit potentials keep their initial values
+        F.view(-1, D)[:, chem] = 0.0
         m = F.view(-1, D)[:, pf].abs().max().item()
         dt = 0.02 / m if m > 0 else 0.0
         X.add_(F, alpha=dt)
@@ -184,7 +220,7 @@ def main():
         "config": dict(cfgdesc, cells=N, faces=NF, dofs_per_cell=D, evolved_steps=evolved,
                        active_set_sizes=nact.tolist(), parallelism="1 GPU"),
         "roofline": {"bound": "hbm", "achieved": gbs64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs64 / HBM_PEAK_GBS, "traffic": None,
+                     "frac": gbs64 / HBM_PEAK_GBS, "traffic": measured_traffic(wl.name),
                      "algorithmic_bytes_per_launch": bytes_alg},
         "fp32_mixed": {"value": rate32, "ms_per_step": ms32, "achieved_gbs": gbs32,
                        "frac": gbs32 / HBM_PEAK_GBS, "speedup_vs_fp64": ms64 / ms32, "rel_err_vs_fp64": rel},

@@ -45,7 +45,10 @@ struct cipher_ctx {
     void *cm = nullptr;
     int32_t *glist = nullptr;
     int32_t *list = nullptr;        // FP32-mixed recomputation list (ncells)
-    int32_t *list_count = nullptr;
+    int32_t *bins = nullptr;        // cells by active-set size (5 * ncells)
+    int32_t *counts = nullptr;      // bin sizes and recomputation count (8)
+    hipStream_t aux[4] = {nullptr, nullptr, nullptr, nullptr};   // default priority
+    hipEvent_t ev_fork = nullptr, ev_join[4] = {nullptr, nullptr, nullptr, nullptr};
     int32_t *err = nullptr;
     double *partial = nullptr;
     double *norm = nullptr;
@@ -57,7 +60,7 @@ struct cipher_ctx {
 };
 
 static void free_mesh(cipher_ctx *c) {
-    void *ptrs[] = {c->cell_off, c->adj, c->bcell_off, c->bface_id, c->cellgeom, c->cm, c->glist, c->list};
+    void *ptrs[] = {c->cell_off, c->adj, c->bcell_off, c->bface_id, c->cellgeom, c->cm, c->glist, c->list, c->bins};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->cell_off = c->adj = c->bcell_off = c->bface_id = nullptr;
@@ -65,6 +68,7 @@ static void free_mesh(cipher_ctx *c) {
     c->cm = nullptr;
     c->glist = nullptr;
     c->list = nullptr;
+    c->bins = nullptr;
     c->ncells = 0;
 }
 
@@ -178,6 +182,16 @@ extern "C" int cipher_ctx_create(const cipher_config *cfg, int precision, cipher
     r |= upload(c, cfg->aniso, cfg->naniso, &d.aniso);
     r |= upload(c, cfg->phase_material, cfg->npf, &d.phase_material);
     r |= upload(c, cfg->interface_list, cfg->npf * cfg->npf, &d.ilist);
+    {
+        // compact [e0, width, m0] per interface type for the pair loops
+        std::vector<double> it(3 * (size_t)std::max(cfg->nf, 1), 0.0);
+        for (int i = 0; i < cfg->nf; i++) {
+            it[3 * i] = cfg->interface[i].e0;
+            it[3 * i + 1] = cfg->interface[i].width;
+            it[3 * i + 2] = cfg->interface[i].m0;
+        }
+        r |= upload(c, it.data(), (int)it.size(), &d.itab);
+    }
     if (r) {
         cipher_ctx_destroy(c);
         return CIPHER_ERR_HIP;
@@ -230,7 +244,18 @@ extern "C" int cipher_ctx_create(const cipher_config *cfg, int precision, cipher
         HIPCHK(hipMemcpy(c->tsv, nanv.data(), sizeof(double) * nanv.size(), hipMemcpyHostToDevice));
     }
     if (c->fold) d.tsv = c->tsv;
-    if (hipMalloc(&c->err, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->list_count, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->partial, sizeof(double) * c->npart) != hipSuccess ||
+    for (int i = 0; i < 4; i++) {
+        if (hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming) != hipSuccess) {
+            cipher_ctx_destroy(c);
+            return fail(CIPHER_ERR_HIP, "stream creation failed");
+        }
+    }
+    if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess) {
+        cipher_ctx_destroy(c);
+        return fail(CIPHER_ERR_HIP, "event creation failed");
+    }
+    if (hipMalloc(&c->err, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->counts, 8 * sizeof(int32_t)) != hipSuccess || hipMalloc(&c->partial, sizeof(double) * c->npart) != hipSuccess ||
         hipMalloc(&c->norm, sizeof(double)) != hipSuccess ||
         hipHostMalloc(&c->h_norm, sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&c->h_err, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
@@ -248,7 +273,12 @@ extern "C" int cipher_ctx_destroy(cipher_ctx *c) {
     free_mesh(c);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->err) (void)hipFree(c->err);
-    if (c->list_count) (void)hipFree(c->list_count);
+    if (c->counts) (void)hipFree(c->counts);
+    for (int i = 0; i < 4; i++) {
+        if (c->aux[i]) (void)hipStreamDestroy(c->aux[i]);
+        if (c->ev_join[i]) (void)hipEventDestroy(c->ev_join[i]);
+    }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->tsv) (void)hipFree(c->tsv);
     if (c->partial) (void)hipFree(c->partial);
     if (c->norm) (void)hipFree(c->norm);
@@ -326,6 +356,7 @@ extern "C" int cipher_upload_mesh(cipher_ctx *c, const cipher_mesh *m) {
     HIPCHK(hipMalloc(&c->cm, cmrec * std::max<long long>(N, 1)));
     HIPCHK(hipMalloc(&c->glist, sizeof(int32_t) * (size_t)(c->L.pf_size + 1) * std::max<long long>(N, 1)));
     HIPCHK(hipMalloc(&c->list, sizeof(int32_t) * std::max<long long>(N, 1)));
+    HIPCHK(hipMalloc(&c->bins, 5 * sizeof(int32_t) * std::max<long long>(N, 1)));
     HIPCHK(hipMemcpy(c->cell_off, off.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->adj, adj.data(), sizeof(int32_t) * adj.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->bcell_off, boff.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice));
@@ -349,8 +380,14 @@ static int rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active)
     L.bcell_off = c->bcell_off;
     L.bface_id = c->bface_id;
     L.cm = c->cm;
-    L.list = c->list;
-    L.list_count = c->list_count;
+    L.redo = c->list;
+    L.bins = c->bins;
+    L.counts = c->counts;
+    for (int i = 0; i < 4; i++) {
+        L.aux[i] = c->aux[i];
+        L.ev_join[i] = c->ev_join[i];
+    }
+    L.ev_fork = c->ev_fork;
     L.ncells = c->ncells;
     if (c->fold) {
         cipher_status f = launch_tsfold(X, c->ncells, c->dc, c->tsv, c->stream);

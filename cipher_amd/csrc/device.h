@@ -27,6 +27,7 @@ struct DevCfg {
     const double *aniso;
     const int32_t *phase_material;
     const int32_t *ilist;
+    const double *itab;       // [e0, width, m0] per interface type
     double junctionpenalty;
     double trate;
     int has_trate;

@@ -16,8 +16,10 @@ struct RhsArgs {
     const int32_t *bcell_off; // ncells+1: boundary faces of each cell
     const int32_t *bface_id;  // boundary label id per boundary-face entry
     const double *cm;         // per-cell [comp(ncp), mob(ncp), tcond]
-    int32_t *list;            // FP32-mixed: cells recomputed in FP64
-    int32_t *list_count;
+    const int32_t *cells;     // cells to process and their (device) count
+    const int32_t *cells_count;
+    int32_t *redo;            // FP32-mixed: cells recomputed in FP64
+    int32_t *redo_count;
     long long ncells;
 };
 
@@ -29,7 +31,11 @@ struct RhsLaunch {
     const double *cellgeom;
     const int32_t *cell_off, *adj, *bcell_off, *bface_id;
     void *cm;
-    int32_t *list, *list_count;
+    int32_t *bins;            // 5 lists of ncells entries (cells by active-set size)
+    int32_t *counts;          // 8 counters: bin sizes, then the recomputation list
+    int32_t *redo;            // ncells entries
+    hipStream_t aux[4];       // forked streams for the concurrent lists
+    hipEvent_t ev_fork, ev_join[4];
     long long ncells;
 };
 

@@ -10,20 +10,22 @@
 //           Laplacian contributions in the same sequence, src/cipher.c:236-389)
 //           fused with the cell update (src/cipher.c:392-664).
 //
-// Mapping: G lanes per cell (G = next power of two >= PF_SIZE), lane g owns
-// active slot g.  Per-cell sums over slots are replayed by every lane in the
-// reference's loop order through group shuffles, so FP64 results follow the
-// reference's rounding sequence.  A block's contiguous X records are staged in
-// LDS with coalesced loads and its F records are assembled in LDS and stored
-// coalesced.  Neighbour records are fetched a batch of faces at a time so the
-// dependent loads of several faces are in flight together.
+// Mapping: the cells are first sorted into lists by active-set size n
+// (k_bin); each list is processed with G = next power of two >= n lanes per
+// cell, lane g owning active slot g, so a two-phase cell takes two lanes
+// whatever PF_SIZE is.  Per-cell sums over slots are replayed by every lane
+// in the reference's loop order through group shuffles, so FP64 results
+// follow the reference's rounding sequence.  A group stages its cell's record
+// in LDS and writes the cell's F record.  Neighbour records are fetched a
+// batch of faces at a time so the dependent loads of several faces are in
+// flight together.
 //
 // FP32-mixed: the face fluxes (Kahan-compensated), interpolants, capillary and
 // rate assembly run in FP32; the implicit site potentials, site fractions,
 // chemical energy, explicit potentials and their rate stay FP64.  Every slot
 // whose active flag is decided by the sign of an FP32 value lying inside its
-// rounding band is recomputed by the FP64 kernel (LIST mode), so the active
-// sets equal the FP64 path's.
+// rounding band sends its cell to a list that the FP64 kernel recomputes, so
+// the active sets equal the FP64 path's.
 #pragma once
 #include "kernels.h"
 
@@ -31,7 +33,8 @@ namespace cipher {
 
 template <int G, class T>
 __device__ __forceinline__ T gsh(T v, int src) {
-    return __shfl(v, src, G);
+    if constexpr (G == 1) return v;
+    else return __shfl(v, src, G);
 }
 
 // interface energy of an interface type at T (src/cipher.c:425, 462-465)
@@ -143,17 +146,11 @@ __device__ __forceinline__ void slot_sitefrac(const DevCfg &c, const cipher_mate
 }
 
 // ---------------------------------------------------------------------------
-// block staging of AoS records: contiguous (block of consecutive cells) or,
-// in LIST mode, each group copies its own cell's record
+// block staging of the AoS records of consecutive cells
 __device__ __forceinline__ void stage_in(double *lds, const double *X, long long c0, int ncell_blk, int D) {
     const long long base = c0 * D;
     const int tot = ncell_blk * D;
     for (int i = threadIdx.x; i < tot; i += blockDim.x) lds[i] = X[base + i];
-}
-__device__ __forceinline__ void stage_out(double *F, const double *lds, long long c0, int ncell_blk, int D) {
-    const long long base = c0 * D;
-    const int tot = ncell_blk * D;
-    for (int i = threadIdx.x; i < tot; i += blockDim.x) F[base + i] = lds[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -273,47 +270,47 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // ---------------------------------------------------------------------------
-// k_rhs: fused face gather + cell right-hand side.  LIST: process the cells
-// of a.list[0 .. *a.list_count) (the FP64 recomputation of FP32-mixed cells
-// whose active flags were inside the rounding band).
-template <int G, int NC, class R, bool LIST>
+// k_rhs: fused face gather + cell right-hand side over a list of cells whose
+// active sets fit G slots (cells are binned by active-set size first, so a
+// two-phase cell occupies two lanes, not PF_SIZE).  Each group of G lanes
+// stages its cell's record in LDS, computes, and writes the cell's F record
+// itself.  In FP32-mixed, cells whose active flags fall inside the FP32
+// rounding band are appended to a.redo for the FP64 pass.
+template <int G, int GMAX, int NC, class R>
 __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg c) {
+    constexpr int RM = (GMAX + G - 1) / G;                 // neighbour entries per lane
     constexpr int ND = NC - 1;
     constexpr int NDV = Cnt<ND>::v;
-    constexpr int CPB = 256 / G;
     constexpr bool MIXED = sizeof(R) == 4;
-    extern __shared__ double lds[];
-    double *ldsX = lds;
-    double *ldsF = lds + CPB * c.D;
-    double *itab = ldsF + CPB * c.D;                       // 3*nf doubles
-    int *ptype = (int *)(itab + 3 * c.nf);                 // CPB*G*G ints
+    extern __shared__ int ptype[];                         // CPB*G*G ints
+    const int CPB = blockDim.x / G;
+    const int D = c.D;
+    const double *itab = c.itab;
     const int lc = threadIdx.x / G;
     const int slot = threadIdx.x & (G - 1);
-    const int D = c.D;
-    for (int i = threadIdx.x; i < c.nf; i += blockDim.x) {
-        itab[3 * i] = c.itf[i].e0;
-        itab[3 * i + 1] = c.itf[i].width;
-        itab[3 * i + 2] = c.itf[i].m0;
-    }
     int *pt = ptype + lc * G * G;
-    const long long limit = LIST ? (long long)*a.list_count : a.ncells;
+    const long long limit = (long long)*a.cells_count;
     const long long stride = (long long)gridDim.x * CPB;
-    for (long long c0 = (long long)blockIdx.x * CPB; c0 < limit; c0 += stride) {
-    const int nblk = (int)min((long long)CPB, limit - c0);
-    long long cell = -1;
-    if (lc < nblk) cell = LIST ? (long long)a.list[c0 + lc] : c0 + lc;
-    if (LIST) {
-        __syncthreads();
-        if (lc < nblk)
-            for (int k = slot; k < D; k += G) ldsX[lc * D + k] = a.X[cell * D + k];
-    } else {
-        stage_in(ldsX, a.X, c0, nblk, D);
-    }
-    for (int i = threadIdx.x; i < nblk * D; i += blockDim.x) ldsF[i] = 0.0;
-    __syncthreads();
-    if (lc < nblk) {
-        const double *x = ldsX + lc * D;
-        double *fr = ldsF + lc * D;
+    // the G lanes of a group share the loop trip count; groups of a wave may
+    // differ, so nothing below synchronises more than the group's wave
+    for (long long j = (long long)blockIdx.x * CPB + lc; j < limit; j += stride) {
+        const long long cell = (long long)a.cells[j];
+        if constexpr (G == 1 && NC == 1) {
+            // single-phase cell without chemistry, conduction or heat terms:
+            // zero phase rate and Laplacian, so F is 0 except the imposed
+            // temperature rate (src/cipher.c:558-568)
+            if (!c.has_tcond && !c.has_theat) {
+                double *Fc = a.F + cell * D;
+                for (int k = 0; k < D; k++) Fc[k] = 0.0;
+                if (c.has_trate) Fc[c.tmo] = 0.0 + c.trate;
+                if (a.active) a.active[cell] = 0u;
+                continue;
+            }
+        }
+        // the group's lanes read the cell's record together (same or
+        // consecutive addresses), so it is served by one or two cache lines
+        const double *x = a.X + cell * D;
+        wave_sync();
         const int n = clamp_count(x[c.aso], c.P);
         const bool on = slot < n;
         const int myid = on ? clamp_id(x[c.aso + 1 + slot], c.npf) : 0xFFFF;
@@ -423,37 +420,61 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
                 }
             }
             // -- in face order: phase Laplacian over the common active phases
+            // the neighbour's active list may be longer than G: lane s holds
+            // its entries s, s+G, s+2G, ...
             int gnext = gsh<G>(guard, 0);
-            double idn = 0.0, phn = 0.0;
-            if (gnext && slot < c.P) {
+            double idn[RM], phn[RM];
+            {
                 const double *xt = a.X + (long long)gsh<G>(nbf, 0) * D;
-                idn = xt[c.aso + 1 + slot];
-                phn = xt[c.pfo + slot];
+                const int cnt = gsh<G>(nbn, 0);
+#pragma unroll
+                for (int r = 0; r < RM; r++) {
+                    const int k = r * G + slot;
+                    const bool ld = gnext && k < cnt;
+                    idn[r] = ld ? xt[c.aso + 1 + k] : 0.0;
+                    phn[r] = ld ? xt[c.pfo + k] : 0.0;
+                }
             }
             #pragma unroll 1
             for (int t = 0; t < nfc; t++) {
                 const int gt = gnext;
-                const double idc = idn, phc = phn;
+                double idc[RM], phc[RM];
+#pragma unroll
+                for (int r = 0; r < RM; r++) { idc[r] = idn[r]; phc[r] = phn[r]; }
                 if (t + 1 < nfc) {
                     gnext = gsh<G>(guard, t + 1);
-                    const int nbt1 = gsh<G>(nbf, t + 1);
-                    if (gnext && slot < c.P) {
-                        const double *xt = a.X + (long long)nbt1 * D;
-                        idn = xt[c.aso + 1 + slot];
-                        phn = xt[c.pfo + slot];
+                    const double *xt = a.X + (long long)gsh<G>(nbf, t + 1) * D;
+                    const int cnt = gsh<G>(nbn, t + 1);
+#pragma unroll
+                    for (int r = 0; r < RM; r++) {
+                        const int k = r * G + slot;
+                        const bool ld = gnext && k < cnt;
+                        idn[r] = ld ? xt[c.aso + 1 + k] : 0.0;
+                        phn[r] = ld ? xt[c.pfo + k] : 0.0;
                     }
                 }
                 if (gt) {
                     const int nbnt = gsh<G>(nbn, t);
-                    const int nbid = slot < nbnt ? clamp_id(idc, c.npf) : 0xFFFE;
-                    const R nbphi = slot < nbnt ? (R)phc : R(0);
                     int pos = -1;
-                    #pragma unroll 1
-                    for (int u = 0; u < nbnt; u++) {
-                        const int v = gsh<G>(nbid, u);
-                        if (v == myid) pos = u;
+#pragma unroll
+                    for (int r = 0; r < RM; r++) {
+                        const int k0 = r * G;
+                        if (k0 < nbnt) {
+                            const int nbid = (k0 + slot < nbnt) ? clamp_id(idc[r], c.npf) : 0xFFFE;
+                            const int ue = min(G, nbnt - k0);
+#pragma unroll
+                            for (int u = 0; u < G; u++)
+                                if (u < ue && gsh<G>(nbid, u) == myid) pos = k0 + u;
+                        }
                     }
-                    const R pv = gsh<G>(nbphi, pos < 0 ? 0 : pos);
+                    R pv = R(0);
+#pragma unroll
+                    for (int r = 0; r < RM; r++) {
+                        if (r * G < nbnt) {
+                            const R cand = gsh<G>((R)phc[r], pos < 0 ? 0 : (pos & (G - 1)));
+                            if (pos >= 0 && pos / G == r) pv = cand;
+                        }
+                    }
                     const R fft = gsh<G>(ff, t), volt = gsh<G>(vol, t);
                     const int sL = gsh<G>((int)selfL, t);
                     if (on && pos >= 0) {
@@ -716,12 +737,12 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
         }
         const int gbase = (threadIdx.x & 63) & ~(G - 1);
         const unsigned long long gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << gbase);
-        if constexpr (MIXED && !LIST) {
+        if constexpr (MIXED) {
             if (__ballot(unsure) & gmask) {
                 // leave this cell to the FP64 pass
                 if (slot == 0) {
-                    const int k = atomicAdd(a.list_count, 1);
-                    a.list[k] = (int32_t)cell;
+                    const int k = atomicAdd(a.redo_count, 1);
+                    a.redo[k] = (int32_t)cell;
                 }
             }
         }
@@ -901,64 +922,178 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
             dmatvec<ND, double>(dprhs, dmdc, wdp);
         }
 
-        // ---- assemble F in LDS ------------------------------------------------
-        if (on) fr[c.pfo + slot] = (double)pfrhs;
+        // ---- write the cell's F record (AS entries and unused slots are 0) ----
+        double *Fc = a.F + cell * D;
+        for (int k = slot; k < c.pfo; k += G) Fc[k] = 0.0;
+        for (int p = slot; p < c.P; p += G) Fc[c.pfo + p] = (p == slot && on) ? (double)pfrhs : 0.0;
         if (slot == 0) {
-            fr[c.tmo] = (double)tmrhs;
+            Fc[c.tmo] = (double)tmrhs;
 #pragma unroll
-            for (int q = 0; q < ND; q++) fr[c.dpo + q] = dprhs[q];
+            for (int q = 0; q < ND; q++) Fc[c.dpo + q] = dprhs[q];
         }
         if constexpr (NC > 1) {
-            if (on) {
+            const int msites = c.SP / ND;
+            for (int p = slot; p < c.P; p += G) {
+                const bool own = (p == slot) && on;
 #pragma unroll
                 for (int s = 0; s < 2; s++) {
-                    if (s >= ns) break;
+                    if (s >= msites) break;
 #pragma unroll
-                    for (int q = 0; q < ND; q++) fr[c.exo + slot * c.SP + s * ND + q] = cprd[s * ND + q];
+                    for (int q = 0; q < ND; q++)
+                        Fc[c.exo + p * c.SP + s * ND + q] = (own && s < ns) ? cprd[s * ND + q] : 0.0;
                 }
             }
         }
+    }  // cell loop
+}
+
+// bin b holds the cells whose active-set size needs 2^b lanes
+__device__ __forceinline__ int lane_bin(int n) {
+    return n <= 1 ? 0 : (n == 2 ? 1 : (n <= 4 ? 2 : (n <= 8 ? 3 : 4)));
+}
+
+// Sort the cells into per-size lists.  A block takes a tile of BIN_TILE
+// consecutive cells, each thread BIN_CPT consecutive cells of it; a block scan
+// of the per-thread counts gives every cell its place, and one atomic per bin
+// per tile reserves the tile's range, so each list keeps cell order within a
+// tile (and the Morton locality of the tiles).
+constexpr int BIN_CPT = 8;
+constexpr int BIN_TILE = 256 * BIN_CPT;
+template <int U>
+__global__ __launch_bounds__(256) void k_bin(const double *__restrict__ X, long long ncells, DevCfg c, int nbins,
+                                              int32_t *__restrict__ bins, long long binstride,
+                                              int32_t *__restrict__ counts) {
+    __shared__ int scan[5][256];
+    __shared__ int base[5];
+    const long long c0 = (long long)blockIdx.x * BIN_TILE + (long long)threadIdx.x * BIN_CPT;
+    int bq[BIN_CPT];
+    int cnt[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < BIN_CPT; k++) {
+        const long long cell = c0 + k;
+        bq[k] = cell < ncells ? lane_bin(clamp_count(X[cell * c.D + c.aso], c.P)) : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < BIN_CPT; k++)
+#pragma unroll
+        for (int b = 0; b < 5; b++) cnt[b] += (bq[k] == b) ? 1 : 0;
+#pragma unroll
+    for (int b = 0; b < 5; b++) scan[b][threadIdx.x] = cnt[b];
+    __syncthreads();
+    // inclusive Hillis-Steele scan of the 256 per-thread counts, per bin
+    for (int off = 1; off < 256; off <<= 1) {
+        int v[5];
+#pragma unroll
+        for (int b = 0; b < 5; b++) v[b] = threadIdx.x >= (unsigned)off ? scan[b][threadIdx.x - off] : 0;
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < 5; b++) scan[b][threadIdx.x] += v[b];
+        __syncthreads();
+    }
+    if (threadIdx.x < (unsigned)nbins) {
+        const int tot = scan[threadIdx.x][255];
+        base[threadIdx.x] = tot ? atomicAdd(&counts[threadIdx.x], tot) : 0;
     }
     __syncthreads();
-    if (LIST) {
-        if (lc < nblk)
-            for (int k = slot; k < D; k += G) a.F[cell * D + k] = ldsF[lc * D + k];
-    } else {
-        stage_out(a.F, ldsF, c0, nblk, D);
+    int pos[5];
+#pragma unroll
+    for (int b = 0; b < 5; b++) pos[b] = base[b] + scan[b][threadIdx.x] - cnt[b];
+#pragma unroll
+    for (int k = 0; k < BIN_CPT; k++) {
+        const int b = bq[k];
+#pragma unroll
+        for (int q = 0; q < 5; q++)
+            if (b == q && q < nbins) bins[q * binstride + pos[q]++] = (int32_t)(c0 + k);
     }
-    }  // block loop
 }
 
 template <int U>
-__global__ void k_zero_i32(int32_t *p) { *p = 0; }
+__global__ void k_fill_i32(int32_t *p, int n, int32_t v) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = v;
+}
+
+// LDS bytes of a k_rhs block of cpb cells
+static inline size_t rhs_lds_bytes(const DevCfg &c, int G, int cpb) {
+    (void)c;
+    return (size_t)cpb * G * G * sizeof(int);
+}
+
+// One k_rhs launch over a device-counted cell list: blocks of 256 threads in
+// a grid sized for the whole mesh (the list length is only known on the
+// device; blocks past it exit at once), striding when the cap is reached.
+template <int G, int GMAX, int NC, class R>
+static void launch_list(const RhsArgs &a0, const DevCfg &c, const int32_t *cells, const int32_t *count,
+                        long long maxcells, hipStream_t st) {
+    RhsArgs a = a0;
+    a.cells = cells;
+    a.cells_count = count;
+    constexpr int T = 256;
+    constexpr int cpb = T / G;
+    long long nb = (maxcells + cpb - 1) / cpb;
+    if (nb > 16384) nb = 16384;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL((k_rhs<G, GMAX, NC, R>), dim3((unsigned)nb), dim3(T), rhs_lds_bytes(c, G, cpb), st, a, c);
+}
+
+// The lists run concurrently: the heaviest (largest active sets, longest
+// per-cell latency) on the caller's stream, the others on forked streams, so
+// one list's tail overlaps the others' work.
+template <int GMAX, int NC, class R>
+static bool launch_bins(const RhsArgs &a, const DevCfg &c, const RhsLaunch &L, hipStream_t st) {
+    int32_t *bins = L.bins;
+    int32_t *counts = L.counts;
+    const long long N = L.ncells;
+    bool ok = hipEventRecord(L.ev_fork, st) == hipSuccess;
+    const hipStream_t *s = L.aux;
+    for (int i = 0; i < 4; i++) ok = ok && hipStreamWaitEvent(s[i], L.ev_fork, 0) == hipSuccess;
+    if (!ok) return false;
+    if constexpr (GMAX >= 16) {
+        launch_list<16, GMAX, NC, R>(a, c, bins + 4 * N, counts + 4, N, st);
+        launch_list<8, GMAX, NC, R>(a, c, bins + 3 * N, counts + 3, N, s[3]);
+    } else if constexpr (GMAX >= 8) {
+        launch_list<8, GMAX, NC, R>(a, c, bins + 3 * N, counts + 3, N, st);
+    }
+    launch_list<4, GMAX, NC, R>(a, c, bins + 2 * N, counts + 2, N, GMAX >= 8 ? s[2] : st);
+    launch_list<2, GMAX, NC, R>(a, c, bins + N, counts + 1, N, s[1]);
+    launch_list<1, GMAX, NC, R>(a, c, bins, counts, N, s[0]);
+    ok = hipGetLastError() == hipSuccess;
+    for (int i = 0; i < 4; i++) {
+        ok = ok && hipEventRecord(L.ev_join[i], s[i]) == hipSuccess;
+        ok = ok && hipStreamWaitEvent(st, L.ev_join[i], 0) == hipSuccess;
+    }
+    return ok;
+}
 
 // ---------------------------------------------------------------------------
 template <int G, int NC>
 static cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st) {
     constexpr int CPB = 256 / G;
-    const long long nb = (L.ncells + CPB - 1) / CPB;
+    const long long N = L.ncells;
+    const long long nb = (N + CPB - 1) / CPB;
     if (nb == 0) return CIPHER_OK;
     const DevCfg &c = *L.cfg;
     const bool need_prep = (NC > 1) || c.has_tcond;
     if (need_prep) {
         const size_t sh = (size_t)CPB * c.D * sizeof(double);
-        hipLaunchKernelGGL((k_prep<G, NC>), dim3((unsigned)nb), dim3(256), sh, st, L.X, (double *)L.cm, L.ncells, c);
+        hipLaunchKernelGGL((k_prep<G, NC>), dim3((unsigned)nb), dim3(256), sh, st, L.X, (double *)L.cm, N, c);
     }
     RhsArgs a;
     a.X = L.X; a.F = L.F; a.active = L.active; a.cellgeom = L.cellgeom;
     a.cell_off = L.cell_off; a.adj = L.adj; a.bcell_off = L.bcell_off; a.bface_id = L.bface_id;
-    a.cm = (const double *)L.cm; a.ncells = L.ncells;
-    a.list = L.list; a.list_count = L.list_count;
-    const size_t sh = 2 * (size_t)CPB * c.D * sizeof(double) + 3 * (size_t)c.nf * sizeof(double)
-                      + (size_t)CPB * G * G * sizeof(int);
+    a.cm = (const double *)L.cm; a.ncells = N;
+    a.cells = nullptr; a.cells_count = nullptr;
+    a.redo = L.redo; a.redo_count = L.counts + 5;
+    // counts[0..4]: bin sizes, counts[5]: FP32-mixed recomputation list
+    hipLaunchKernelGGL(k_fill_i32<0>, dim3(1), dim3(64), 0, st, L.counts, 8, 0);
+    const int nbins = G >= 16 ? 5 : (G >= 8 ? 4 : 3);
+    hipLaunchKernelGGL(k_bin<0>, dim3((unsigned)((N + BIN_TILE - 1) / BIN_TILE)), dim3(256), 0, st, L.X, N, c, nbins,
+                       L.bins, N, L.counts);
     if (precision == CIPHER_FP32_MIXED) {
-        hipLaunchKernelGGL(k_zero_i32<0>, dim3(1), dim3(1), 0, st, L.list_count);
-        hipLaunchKernelGGL((k_rhs<G, NC, float, false>), dim3((unsigned)nb), dim3(256), sh, st, a, c);
+        if (!launch_bins<G, NC, float>(a, c, L, st)) return CIPHER_ERR_HIP;
         // FP64 recomputation of the cells whose flags sat in the FP32 band
-        const long long rb = nb < 512 ? nb : 512;
-        hipLaunchKernelGGL((k_rhs<G, NC, double, true>), dim3((unsigned)rb), dim3(256), sh, st, a, c);
+        launch_list<G, G, NC, double>(a, c, L.redo, L.counts + 5, N, st);
     } else {
-        hipLaunchKernelGGL((k_rhs<G, NC, double, false>), dim3((unsigned)nb), dim3(256), sh, st, a, c);
+        if (!launch_bins<G, NC, double>(a, c, L, st)) return CIPHER_ERR_HIP;
     }
     return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
 }
