@@ -102,6 +102,26 @@ class Mesh(C.Structure):
                 ("pad_", C.c_int32)]
 
 
+class TsParams(C.Structure):
+    _fields_ = [("atol", C.c_double), ("rtol", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
+                ("safety", C.c_double), ("reject_safety", C.c_double), ("clip_lo", C.c_double),
+                ("clip_hi", C.c_double), ("use_graph", C.c_int32), ("pad_", C.c_int32)]
+
+
+class TsCtrl(C.Structure):
+    _fields_ = [("err_prev", C.c_double), ("safety", C.c_double), ("rejected_last", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
+class TsStats(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("rejected", C.c_int64), ("rhs_evals", C.c_int64), ("t", C.c_double),
+                ("dt", C.c_double), ("err", C.c_double)]
+
+
+def ts_params(atol, rtol, dt_min, dt_max, safety=0.9, reject_safety=0.5, clip=(0.1, 10.0), use_graph=False):
+    return TsParams(atol, rtol, dt_min, dt_max, safety, reject_safety, clip[0], clip[1], 1 if use_graph else 0, 0)
+
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcipher_amd.so")
 _lib = None
@@ -125,6 +145,12 @@ def _declare(lib):
         "cipher_wrms_norm": (i32, [vp, vp, vp, i64, dbl, dbl, C.POINTER(C.c_double)]),
         "cipher_axpy_n": (i32, [vp, i64, vp, vp, i32, C.POINTER(C.c_double), C.POINTER(vp)]),
         "cipher_last_error": (C.c_char_p, []),
+        "cipher_ts_adapt": (i32, [C.POINTER(TsParams), C.POINTER(TsCtrl), dbl, dbl, C.POINTER(C.c_int),
+                                  C.POINTER(C.c_double)]),
+        "cipher_ts_setup": (i32, [vp, C.POINTER(TsParams)]),
+        "cipher_ts_attempt": (i32, [vp, vp, dbl, C.POINTER(C.c_double)]),
+        "cipher_ts_accept": (i32, [vp, vp]),
+        "cipher_ts_solve": (i32, [vp, vp, dbl, dbl, C.POINTER(C.c_double), i64, C.POINTER(TsStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -136,7 +162,8 @@ def _declare(lib):
 EXPORTED = ("cipher_layout_of", "cipher_ctx_create", "cipher_ctx_destroy", "cipher_ctx_layout",
             "cipher_set_precision", "cipher_set_stream", "cipher_set_temperature_rate",
             "cipher_upload_mesh", "cipher_rhs", "cipher_rhs_active", "cipher_poststep",
-            "cipher_setup_problem", "cipher_wrms_norm", "cipher_axpy_n", "cipher_last_error")
+            "cipher_setup_problem", "cipher_wrms_norm", "cipher_axpy_n", "cipher_last_error",
+            "cipher_ts_adapt", "cipher_ts_setup", "cipher_ts_attempt", "cipher_ts_accept", "cipher_ts_solve")
 
 
 def load_library(path=None):

@@ -13,7 +13,7 @@
 #include <string>
 #include <vector>
 
-#include "kernels.h"
+#include "ctx.h"
 
 using namespace cipher;
 
@@ -23,6 +23,7 @@ static int fail(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
+int cipher_fail(int code, const std::string &msg) { return fail(code, msg); }
 
 #define HIPCHK(expr)                                                                     \
     do {                                                                                 \
@@ -30,34 +31,6 @@ static int fail(int code, const std::string &msg) {
         if (e_ != hipSuccess)                                                            \
             return fail(CIPHER_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
     } while (0)
-
-struct cipher_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int precision = CIPHER_FP64;
-    cipher_layout L{};
-    DevCfg dc{};
-    std::vector<void *> allocs;   // config tables
-    // mesh
-    long long ncells = 0;
-    int32_t *cell_off = nullptr, *adj = nullptr, *bcell_off = nullptr, *bface_id = nullptr;
-    double *cellgeom = nullptr;
-    void *cm = nullptr;
-    int32_t *glist = nullptr;
-    int32_t *list = nullptr;        // FP32-mixed recomputation list (ncells)
-    int32_t *bins = nullptr;        // cells by active-set size (5 * ncells)
-    int32_t *counts = nullptr;      // bin sizes and recomputation count (8)
-    hipStream_t aux[4] = {nullptr, nullptr, nullptr, nullptr};   // default priority
-    hipEvent_t ev_fork = nullptr, ev_join[4] = {nullptr, nullptr, nullptr, nullptr};
-    int32_t *err = nullptr;
-    double *partial = nullptr;
-    double *norm = nullptr;
-    double *tsv = nullptr;          // folded T-series (ntseries + 1)
-    bool fold = false;
-    double *h_norm = nullptr;
-    int32_t *h_err = nullptr;
-    int npart = 1024;
-};
 
 static void free_mesh(cipher_ctx *c) {
     void *ptrs[] = {c->cell_off, c->adj, c->bcell_off, c->bface_id, c->cellgeom, c->cm, c->glist, c->list, c->bins};
@@ -270,6 +243,7 @@ extern "C" int cipher_ctx_destroy(cipher_ctx *c) {
     if (!c) return CIPHER_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     else (void)hipDeviceSynchronize();
+    cipher_ts_free(c);
     free_mesh(c);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->err) (void)hipFree(c->err);
@@ -297,18 +271,21 @@ extern "C" int cipher_ctx_layout(const cipher_ctx *c, cipher_layout *out) {
 extern "C" int cipher_set_precision(cipher_ctx *c, int precision) {
     if (!c) return fail(CIPHER_ERR_ARG, "null ctx");
     if (precision != CIPHER_FP64 && precision != CIPHER_FP32_MIXED) return fail(CIPHER_ERR_ARG, "precision");
+    if (c->precision != precision) cipher_ts_invalidate(c);
     c->precision = precision;
     return CIPHER_OK;
 }
 
 extern "C" int cipher_set_stream(cipher_ctx *c, void *stream) {
     if (!c) return fail(CIPHER_ERR_ARG, "null ctx");
+    if (c->stream != (hipStream_t)stream) cipher_ts_invalidate(c);
     c->stream = (hipStream_t)stream;
     return CIPHER_OK;
 }
 
 extern "C" int cipher_set_temperature_rate(cipher_ctx *c, int has_rate, double rate) {
     if (!c) return fail(CIPHER_ERR_ARG, "null ctx");
+    cipher_ts_invalidate(c);
     c->dc.has_trate = has_rate ? 1 : 0;
     c->dc.trate = rate;
     return CIPHER_OK;
@@ -346,6 +323,7 @@ extern "C" int cipher_upload_mesh(cipher_ctx *c, const cipher_mesh *m) {
     std::vector<int32_t> bid(std::max<long long>(boff[N], 1));
     std::vector<int32_t> bpos(boff.begin(), boff.end() - 1);
     for (int k = 0; k < m->nbfaces; k++) bid[bpos[m->bface_cell[k]]++] = m->bface_id[k];
+    cipher_ts_free(c);                 // stage vectors are sized for the old mesh
     free_mesh(c);
     size_t cmrec = prep_record_bytes(c->dc.ncp);
     HIPCHK(hipMalloc(&c->cell_off, sizeof(int32_t) * (N + 1)));
@@ -366,7 +344,7 @@ extern "C" int cipher_upload_mesh(cipher_ctx *c, const cipher_mesh *m) {
     return CIPHER_OK;
 }
 
-static int rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active) {
+int cipher_rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active) {
     if (!c || !X || !F) return fail(CIPHER_ERR_ARG, "null argument");
     if (!c->cell_off) return fail(CIPHER_ERR_STATE, "no mesh uploaded");
     RhsLaunch L;
@@ -400,16 +378,16 @@ static int rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active)
 
 extern "C" int cipher_rhs(cipher_ctx *c, double t, const double *X, double *F) {
     (void)t;
-    return rhs_impl(c, X, F, nullptr);
+    return cipher_rhs_impl(c, X, F, nullptr);
 }
 
 extern "C" int cipher_rhs_active(cipher_ctx *c, double t, const double *X, double *F, uint32_t *active) {
     (void)t;
     if (!active) return fail(CIPHER_ERR_ARG, "null active buffer");
-    return rhs_impl(c, X, F, active);
+    return cipher_rhs_impl(c, X, F, active);
 }
 
-static int post_common(cipher_ctx *c, double *X, const int32_t *phase, bool setup) {
+int cipher_post_common(cipher_ctx *c, double *X, const int32_t *phase, bool setup) {
     if (!c || !X) return fail(CIPHER_ERR_ARG, "null argument");
     if (!c->cell_off) return fail(CIPHER_ERR_STATE, "no mesh uploaded");
     PostLaunch L;
@@ -431,11 +409,11 @@ static int post_common(cipher_ctx *c, double *X, const int32_t *phase, bool setu
     return CIPHER_OK;
 }
 
-extern "C" int cipher_poststep(cipher_ctx *c, double *X) { return post_common(c, X, nullptr, false); }
+extern "C" int cipher_poststep(cipher_ctx *c, double *X) { return cipher_post_common(c, X, nullptr, false); }
 
 extern "C" int cipher_setup_problem(cipher_ctx *c, const int32_t *phase, double *X) {
     if (!phase) return fail(CIPHER_ERR_ARG, "null phase labels");
-    return post_common(c, X, phase, true);
+    return cipher_post_common(c, X, phase, true);
 }
 
 extern "C" int cipher_wrms_norm(cipher_ctx *c, const double *U, const double *Y, int64_t n, double atol, double rtol,

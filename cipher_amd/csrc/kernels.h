@@ -59,5 +59,7 @@ cipher_status launch_wrms(const double *U, const double *Y, long long n, double 
                           double *partial, int npart, double *out, hipStream_t st);
 cipher_status launch_axpy_n(long long n, double *Y, const double *X, int nk, const double *a,
                             const double *const *K, hipStream_t st);
+cipher_status launch_axpy_dt(long long n, double *Y, const double *X, const double *dt_dev, int nk, const double *A,
+                             const double *const *K, hipStream_t st);
 
 }  // namespace cipher

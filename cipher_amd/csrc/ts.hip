@@ -104,4 +104,42 @@ cipher_status launch_axpy_n(long long n, double *Y, const double *X, int nk, con
     return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
 }
 
+// Y = X + (((dt*A0) K0 + (dt*A1) K1) + ...) with dt read from device memory,
+// so a captured stage sequence stays valid when the step size changes
+template <int NK>
+__global__ __launch_bounds__(256) void k_axpy_dt(long long n, double *__restrict__ Y, const double *__restrict__ X,
+                                                 const double *__restrict__ dtp, AxpyArgs p) {
+    const double dt = dtp[0];
+    double a[NK];
+#pragma unroll
+    for (int j = 0; j < NK; j++) a[j] = dt * p.a[j];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double t = a[0] * p.K[0][i];
+#pragma unroll
+        for (int j = 1; j < NK; j++) t += a[j] * p.K[j][i];
+        Y[i] = X[i] + t;
+    }
+}
+
+cipher_status launch_axpy_dt(long long n, double *Y, const double *X, const double *dt_dev, int nk, const double *A,
+                             const double *const *K, hipStream_t st) {
+    if (nk < 1 || nk > 4) return CIPHER_ERR_ARG;
+    AxpyArgs p;
+    for (int j = 0; j < 8; j++) {
+        p.K[j] = j < nk ? K[j] : nullptr;
+        p.a[j] = j < nk ? A[j] : 0.0;
+    }
+    long long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    switch (nk) {
+    case 1: hipLaunchKernelGGL((k_axpy_dt<1>), dim3((unsigned)blocks), dim3(256), 0, st, n, Y, X, dt_dev, p); break;
+    case 2: hipLaunchKernelGGL((k_axpy_dt<2>), dim3((unsigned)blocks), dim3(256), 0, st, n, Y, X, dt_dev, p); break;
+    case 3: hipLaunchKernelGGL((k_axpy_dt<3>), dim3((unsigned)blocks), dim3(256), 0, st, n, Y, X, dt_dev, p); break;
+    default: hipLaunchKernelGGL((k_axpy_dt<4>), dim3((unsigned)blocks), dim3(256), 0, st, n, Y, X, dt_dev, p); break;
+    }
+    return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
+}
+
 }  // namespace cipher

@@ -105,6 +105,33 @@ class Context:
         abi.check(self.lib.cipher_axpy_n(self.h, X.numel(), _ptr(Y), _ptr(X), nk, a, ks), "cipher_axpy_n")
         return Y
 
+    # ---- adaptive RK (TSRK 3bs + step controller, src/cipher.c:1097-1115) ----
+    def ts_setup(self, params):
+        self.ts_params = params
+        abi.check(self.lib.cipher_ts_setup(self.h, C.byref(params)), "cipher_ts_setup")
+
+    def ts_attempt(self, X, dt):
+        self._check_state(X, "X")
+        self._stream()
+        err = C.c_double()
+        abi.check(self.lib.cipher_ts_attempt(self.h, _ptr(X), dt, C.byref(err)), "cipher_ts_attempt")
+        return err.value
+
+    def ts_accept(self, X):
+        self._check_state(X, "X")
+        self._stream()
+        abi.check(self.lib.cipher_ts_accept(self.h, _ptr(X)), "cipher_ts_accept")
+
+    def ts_solve(self, X, t0, t_end, dt, max_steps=0):
+        """Integrate X in place; returns (stats, next dt)."""
+        self._check_state(X, "X")
+        self._stream()
+        st = abi.TsStats()
+        d = C.c_double(dt)
+        abi.check(self.lib.cipher_ts_solve(self.h, _ptr(X), t0, t_end, C.byref(d), max_steps, C.byref(st)),
+                  "cipher_ts_solve")
+        return st, d.value
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.cipher_ctx_destroy(self.h)
@@ -122,6 +149,16 @@ def RHSFunction(ts, ftime, X, F, user):
     user is a Context; X and F are device state tensors."""
     user.rhs(X, F, t=ftime)
     return 0
+
+
+def ts_adapt(params, ctrl, dt, err):
+    """The library's step-size decision (host only): (accept, dt_next)."""
+    lib = abi.load_library()
+    acc = C.c_int()
+    nxt = C.c_double()
+    abi.check(lib.cipher_ts_adapt(C.byref(params), C.byref(ctrl), dt, err, C.byref(acc), C.byref(nxt)),
+              "cipher_ts_adapt")
+    return bool(acc.value), nxt.value
 
 
 def PostStep(ts):

@@ -251,6 +251,64 @@ int cipher_wrms_norm(cipher_ctx *ctx, const double *U_dev, const double *Y_dev,
 int cipher_axpy_n(cipher_ctx *ctx, int64_t n, double *Y_dev, const double *X_dev,
                   int nk, const double *a, const double *const *K_dev);
 
+/* ---- adaptive explicit Runge-Kutta integration on device vectors ----------
+ * The integrator CIPHER configures in PETSc (TSRK, default type 3bs =
+ * Bogacki-Shampine 3(2); TSADAPTDSP; TSSetTolerances; TSAdaptSetStepLimits;
+ * src/cipher.c:1097-1115) and its time loop with an exact final time
+ * (TS_EXACTFINALTIME_MATCHSTEP, src/cipher.c:1589-1606), with PostStep's
+ * active-set update after every accepted step.  The stage values use
+ * cipher_rhs, the error is cipher_wrms_norm of the third- against the
+ * embedded second-order solution. */
+typedef struct cipher_ts_params {
+    double atol, rtol;          /* TSSetTolerances, src/cipher.c:1112 (abstol/reltol, src/init.c:1286-1291) */
+    double dt_min, dt_max;      /* TSAdaptSetStepLimits, src/cipher.c:1111 (timestepmin/max) */
+    double safety;              /* step-ratio safety factor (0.9) */
+    double reject_safety;       /* extra factor after consecutive rejections (0.5) */
+    double clip_lo, clip_hi;    /* bounds of the step ratio (0.1, 10) */
+    int32_t use_graph;          /* replay each attempt from a captured HIP graph */
+    int32_t pad_;
+} cipher_ts_params;
+
+/* Controller memory carried between steps (zero-initialise before a run). */
+typedef struct cipher_ts_ctrl {
+    double err_prev;            /* error norm of the last accepted step, 0 if none */
+    double safety;              /* current safety factor, 0 = params.safety */
+    int32_t rejected_last;      /* the previous attempt was rejected */
+    int32_t pad_;
+} cipher_ts_ctrl;
+
+typedef struct cipher_ts_stats {
+    int64_t steps;              /* accepted steps */
+    int64_t rejected;           /* rejected attempts */
+    int64_t rhs_evals;
+    double t;                   /* time reached */
+    double dt;                  /* next step size */
+    double err;                 /* error norm of the last attempt */
+} cipher_ts_stats;
+
+/* Host-only step-size decision for an attempt of size dt with error norm err
+ * (no device work): *accept = err <= 1 or dt at dt_min; *dt_next from the
+ * PI controller with an arctan limiter, clipped and clamped to the limits.
+ * CIPHER_ERR_NONFINITE for a non-finite err at the minimum step. */
+int cipher_ts_adapt(const cipher_ts_params *p, cipher_ts_ctrl *st, double dt, double err,
+                    int *accept, double *dt_next);
+
+/* Allocate the stage vectors for the uploaded mesh and reset the controller.
+ * Call again after cipher_upload_mesh. */
+int cipher_ts_setup(cipher_ctx *ctx, const cipher_ts_params *p);
+
+/* One attempt of size dt from X_dev (unchanged): four right-hand sides, the
+ * new and embedded solutions, the error norm (host, synchronises). */
+int cipher_ts_attempt(cipher_ctx *ctx, const double *X_dev, double dt, double *err_out);
+
+/* Copy the last attempt's solution into X_dev and run cipher_poststep on it. */
+int cipher_ts_accept(cipher_ctx *ctx, double *X_dev);
+
+/* Integrate X_dev from t0 to t_end (exact final time) or for max_steps
+ * accepted steps; *dt_inout: first step size in, next step size out. */
+int cipher_ts_solve(cipher_ctx *ctx, double *X_dev, double t0, double t_end, double *dt_inout,
+                    int64_t max_steps, cipher_ts_stats *stats);
+
 /* Message of the last failing call on this thread. */
 const char *cipher_last_error(void);
 

@@ -262,3 +262,51 @@ def chem(thermal=False, sources=True):
     if key not in _cache:
         _cache[key] = W.Workload(chem_test_yaml(sources=sources, thermal=thermal), "chem")
     return _cache[key]
+
+
+# ---- CPU restatement of the adaptive RK driver (for trajectory parity) ----
+BS3_A2, BS3_A3 = 1.0 / 2.0, 3.0 / 4.0
+BS3_B = (2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0)
+BS3_BH = (7.0 / 24.0, 1.0 / 4.0, 1.0 / 3.0, 1.0 / 8.0)
+
+
+def cpu_attempt(wl, X, dt, atol, rtol, rhs=None):
+    """One Bogacki-Shampine 3(2) attempt with the oracle RHS, in the stage
+    arithmetic of the device integrator: Y = X + ((a0 K0 + a1 K1) + ...),
+    a_j = dt * A_j.  Returns (new solution, embedded solution, error norm)."""
+    f = rhs or (lambda Y: oracle.rhs(wl.cfg, wl.mesh, Y))
+    K1 = f(X)
+    K2 = f(X + (dt * BS3_A2) * K1)
+    K3 = f(X + (dt * BS3_A3) * K2)
+    a = [dt * b for b in BS3_B]
+    Xn = X + ((a[0] * K1 + a[1] * K2) + a[2] * K3)
+    K4 = f(Xn)
+    ah = [dt * b for b in BS3_BH]
+    Xh = X + (((ah[0] * K1 + ah[1] * K2) + ah[2] * K3) + ah[3] * K4)
+    return Xn, Xh, oracle.wrms_norm(Xn, Xh, atol, rtol)
+
+
+def cpu_solve(wl, X, t0, t_end, dt, params, max_steps=0):
+    """The device integrator's loop (exact final time, PostStep after each
+    accepted step) on the CPU, with the library's host step controller."""
+    from cipher_amd import abi
+    from cipher_amd.runtime import ts_adapt
+    ctrl = abi.TsCtrl()
+    t, steps, rejected, dts = t0, 0, 0, []
+    X = np.array(X, dtype=np.float64)
+    while t < t_end and (max_steps <= 0 or steps < max_steps):
+        dt_try = t_end - t if t + dt > t_end else dt
+        if not dt_try > 0:
+            break
+        Xn, _, err = cpu_attempt(wl, X, dt_try, params.atol, params.rtol)
+        acc, dt_next = ts_adapt(params, ctrl, dt_try, err)
+        dts.append((dt_try, err, acc))
+        if acc:
+            X, rc = oracle.poststep(wl.cfg, wl.mesh, Xn)
+            assert rc == 0
+            t = t_end if dt_try == t_end - t else t + dt_try
+            steps += 1
+        else:
+            rejected += 1
+        dt = dt_next
+    return X, dict(steps=steps, rejected=rejected, t=t, dt=dt, history=dts)

@@ -1,0 +1,203 @@
+"""The adaptive Runge-Kutta integrator: host step controller (CPU), the
+Bogacki-Shampine tableau (order of the CPU restatement), and the device
+integrator against the CPU restatement (GPU)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from cipher_amd import abi
+from tests import helpers as H
+
+
+def _params(**kw):
+    d = dict(atol=1e-4, rtol=1e-4, dt_min=1e-6, dt_max=10.0)
+    d.update(kw)
+    return abi.ts_params(**d)
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import os
+    if not os.path.exists(abi.LIB_PATH):
+        from cipher_amd.csrc import build
+        build.build()
+    return abi.load_library()
+
+
+def test_controller_accepts_grows_rejects_shrinks(lib):
+    from cipher_amd.runtime import ts_adapt
+    p = _params()
+    acc, nxt = ts_adapt(p, abi.TsCtrl(), 0.1, 0.25)
+    assert acc and nxt > 0.1
+    acc, nxt = ts_adapt(p, abi.TsCtrl(), 0.1, 4.0)
+    assert not acc and nxt < 0.1
+    # smaller error, larger next step (elementary controller, no history)
+    prev = 0.0
+    for e in (8.0, 2.0, 1.0, 0.5, 0.1, 0.01):
+        _, nxt = ts_adapt(p, abi.TsCtrl(), 0.1, e)
+        assert nxt >= prev
+        prev = nxt
+    # the arctan limiter bounds a shrink to 1 + atan(-1) (times the safety);
+    # a zero error takes the largest ratio, then the limits clamp
+    _, nxt = ts_adapt(p, abi.TsCtrl(), 0.1, 0.0)
+    assert nxt == pytest.approx(0.1 * p.safety * p.clip_hi)
+    _, nxt = ts_adapt(p, abi.TsCtrl(), 0.1, 1e300)
+    assert nxt == pytest.approx(0.1 * p.safety * (1.0 + math.atan(-1.0)), rel=1e-9)
+    _, nxt = ts_adapt(_params(dt_max=0.15), abi.TsCtrl(), 0.1, 1e-8)
+    assert nxt == 0.15
+
+
+def test_controller_minimum_step_and_nonfinite(lib):
+    from cipher_amd.runtime import ts_adapt
+    p = _params(dt_min=1e-3)
+    acc, nxt = ts_adapt(p, abi.TsCtrl(), 1e-3, 50.0)
+    assert acc and nxt == 1e-3                     # accepted at the minimum step
+    acc, nxt = ts_adapt(p, abi.TsCtrl(), 0.1, float("nan"))
+    assert not acc and nxt == pytest.approx(0.01)
+    with pytest.raises(abi.CipherError) as e:
+        ts_adapt(p, abi.TsCtrl(), 1e-3, float("inf"))
+    assert e.value.code == 4
+
+
+def test_controller_history_and_consecutive_rejections(lib):
+    from cipher_amd.runtime import ts_adapt
+    p = _params()
+    # with an accepted history, a rising error damps the growth (PI action)
+    c1 = abi.TsCtrl()
+    ts_adapt(p, c1, 0.1, 0.1)
+    _, n_hist = ts_adapt(p, c1, 0.1, 0.5)
+    _, n_fresh = ts_adapt(p, abi.TsCtrl(), 0.1, 0.5)
+    assert n_hist < n_fresh
+    # a second rejection in a row shortens more aggressively
+    c2 = abi.TsCtrl()
+    _, first = ts_adapt(p, c2, 0.1, 3.0)
+    _, second = ts_adapt(p, c2, 0.1, 3.0)
+    assert second == pytest.approx(first * p.reject_safety, rel=1e-12)
+    assert c2.rejected_last == 1
+
+
+def test_bogacki_shampine_orders():
+    """On the oracle RHS with frozen active sets: the embedded error estimate
+    is O(dt^3) and the step's local error O(dt^4) (halving dt divides them by
+    ~8 and ~16), which fixes the tableau."""
+    wl = H.gg2d()
+    X = H.evolve(wl, H.initial_state(wl), steps=4)
+    F = oracle.rhs(wl.cfg, wl.mesh, X)
+    dt0 = 0.01 / np.abs(F).max()
+    est, loc = [], []
+    for k in range(3):
+        dt = dt0 / 2 ** k
+        Xn, Xh, _ = H.cpu_attempt(wl, X, dt, 1e-4, 1e-4)
+        # reference: 16 substeps of the same scheme
+        Y = X.copy()
+        for _ in range(16):
+            Y, _, _ = H.cpu_attempt(wl, Y, dt / 16, 1e-4, 1e-4)
+        est.append(np.abs(Xn - Xh).max())
+        loc.append(np.abs(Xn - Y).max())
+    r_est = est[1] / est[2]
+    r_loc = loc[1] / loc[2]
+    assert 6.0 < r_est < 10.0, est
+    assert 12.0 < r_loc < 20.0, loc
+
+
+pytestmark_gpu = pytest.mark.gpu
+
+
+def _ctx(wl, params):
+    from cipher_amd.runtime import Context
+    ctx = Context(wl.cfg)
+    ctx.upload_mesh(wl.mesh)
+    ctx.ts_setup(params)
+    return ctx
+
+
+def _dev(ctx, X):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(X)).to(ctx.device)
+
+
+def _host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["gg2d", "ternary"])
+def test_attempt_matches_cpu(name):
+    wl = getattr(H, name)()
+    X = H.evolve(wl, H.initial_state(wl), steps=3)
+    F = oracle.rhs(wl.cfg, wl.mesh, X)
+    L = wl.layout
+    Fv = F.reshape(-1, L["ndof"])
+    dt = 0.02 / np.abs(Fv[:, L["pf_offset"]:L["pf_offset"] + L["pf_size"]]).max()
+    dt = min(dt, 0.02 / (np.abs(Fv) / np.maximum(np.abs(X.reshape(-1, L["ndof"])), 1.0)).max())
+    p = _params()
+    ctx = _ctx(wl, p)
+    print(f"[ts attempt] {name} dt={dt:.3e}", flush=True)
+    Xd = _dev(ctx, X)
+    err_g = ctx.ts_attempt(Xd, dt)
+    Xn, _, err_c = H.cpu_attempt(wl, X, dt, p.atol, p.rtol)
+    assert err_g == pytest.approx(err_c, rel=1e-8)
+    ctx.ts_accept(Xd)
+    Xo, rc = oracle.poststep(wl.cfg, wl.mesh, Xn)
+    m = H.state_mask(Xo, L, wl.cfg)
+    Xg = _host(Xd)
+    assert np.array_equal(H.state_mask(Xg, L, wl.cfg), m)
+    errs = H.field_errors(np.where(m, Xg, 0.0), np.where(m, Xo, 0.0), L)
+    print("   state errors", errs, flush=True)
+    for f, e in errs.items():
+        assert e <= 1e-10, f
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["gg2d", "gg3d_amr"])
+def test_trajectory_matches_cpu(name):
+    """Full adaptive trajectory: accepted/rejected counts, step sizes and the
+    final phase fields agree with the CPU restatement (same controller)."""
+    wl = getattr(H, name)()
+    X0 = H.initial_state(wl)
+    p = _params(atol=1e-3, rtol=1e-3, dt_min=1e-4, dt_max=2.0)
+    t_end = 4.0 if name == "gg2d" else 1.5
+    Xc, info = H.cpu_solve(wl, X0, 0.0, t_end, 1e-4, p)
+    ctx = _ctx(wl, p)
+    Xd = _dev(ctx, X0)
+    print(f"[ts solve] {name} cpu steps={info['steps']} rejected={info['rejected']}", flush=True)
+    st, dt_next = ctx.ts_solve(Xd, 0.0, t_end, 1e-4)
+    print(f"   gpu steps={st.steps} rejected={st.rejected} t={st.t} dt={st.dt}", flush=True)
+    assert (st.steps, st.rejected) == (info["steps"], info["rejected"])
+    assert st.t == info["t"]
+    assert dt_next == pytest.approx(info["dt"], rel=1e-8)
+    Xg = _host(Xd)
+    L = wl.layout
+    m = H.state_mask(Xc, L, wl.cfg)
+    assert np.array_equal(H.state_mask(Xg, L, wl.cfg), m)
+    errs = H.field_errors(np.where(m, Xg, 0.0), np.where(m, Xc, 0.0), L)
+    print("   final state errors", errs, flush=True)
+    assert errs["PF"] <= 1e-9
+
+
+@pytest.mark.gpu
+def test_graph_replays_match_eager():
+    """The captured attempt, replayed over many steps with eager work between
+    the replays, gives the same trajectory bits as eager launches."""
+    wl = H.gg3d_amr()
+    X0 = H.initial_state(wl)
+    pe = _params(atol=1e-3, rtol=1e-3, dt_min=1e-4, dt_max=2.0)
+    pg = _params(atol=1e-3, rtol=1e-3, dt_min=1e-4, dt_max=2.0, use_graph=True)
+    ce, cg = _ctx(wl, pe), _ctx(wl, pg)
+    Xe, Xg = _dev(ce, X0), _dev(cg, X0)
+    se, de = ce.ts_solve(Xe, 0.0, 1.0, 1e-4)
+    sg, dg = cg.ts_solve(Xg, 0.0, 1.0, 1e-4)
+    print(f"[ts graph] eager {se.steps}/{se.rejected} graph {sg.steps}/{sg.rejected}", flush=True)
+    assert (se.steps, se.rejected, se.t, de) == (sg.steps, sg.rejected, sg.t, dg)
+    assert np.array_equal(_host(Xe), _host(Xg))
+    # the FP32-mixed path changes the captured launches: recaptured
+    cg.set_precision(abi.FP32_MIXED)
+    ce.set_precision(abi.FP32_MIXED)
+    se2, _ = ce.ts_solve(Xe, 1.0, 1.5, de)
+    sg2, _ = cg.ts_solve(Xg, 1.0, 1.5, dg)
+    assert (se2.steps, se2.rejected) == (sg2.steps, sg2.rejected)
+    assert np.array_equal(_host(Xe), _host(Xg))
