@@ -246,17 +246,19 @@ __global__ __launch_bounds__(256) void k_prep(const double *__restrict__ X, doub
 }
 
 // ---------------------------------------------------------------------------
-// interface constants staged in LDS: [e0, width, m0] per interface type
+// interface energy / mobility of a pair from the compact [e0, width, m0]
+// table; an activation factor exp(Q(T)/RT) is evaluated in FP64 (Q/RT is
+// often 10-20, so an FP32 exponent would lose that many ulps)
 template <class R>
-__device__ __forceinline__ R pair_e(const DevCfg &c, const double *itab, int it, R T) {
-    if (c.has_iact) return itf_energy<R>(c, it, T);
+__device__ __forceinline__ R pair_e(const DevCfg &c, const double *itab, int it, double T) {
+    if (c.has_iact) return (R)itf_energy<double>(c, it, T);
     R e = R(1);
     e *= (R)itab[3 * it];
     return e;
 }
 template <class R>
-__device__ __forceinline__ R pair_m(const DevCfg &c, const double *itab, int it, R T) {
-    if (c.has_iact) return itf_mobility<R>(c, it, T);
+__device__ __forceinline__ R pair_m(const DevCfg &c, const double *itab, int it, double T) {
+    if (c.has_iact) return (R)itf_mobility<double>(c, it, T);
     R m = R(1);
     m *= (R)itab[3 * it + 2];
     return m;
@@ -538,17 +540,23 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
         R hh, h;
         interpolant<G, R>(c, n, phi, hh, h);
         if (!on) { h = R(0); hh = R(0); }
+        // FP64 interpolants for the chemical driving force: chemical energies
+        // (~1e9 J/m^3) nearly cancel between phases, beyond FP32's reach
+        double hhd, hd;
+        if constexpr (MIXED) {
+            interpolant<G, double>(c, n, phid, hhd, hd);
+            if (!on) { hd = 0.0; hhd = 0.0; }
+        } else {
+            hhd = hh;
+            hd = h;
+        }
         const int mat = on ? c.phase_material[myid] : 0;
         const cipher_material &m = c.mat[mat];
         const int ns = on ? m.nsites : 0;
 
         // site fractions of the slot's phase, FP64 (same arithmetic as k_prep)
         double yd[2 * NC], spimd[2 * NDV];
-        slot_sitefrac<G, NC>(c, m, on, ns, n, myid, (double)h, x, slot, Td, spimd, yd);
-        R y[2 * NC];
-#pragma unroll
-        for (int k = 0; k < 2 * NC; k++) y[k] = (R)yd[k];
-
+        slot_sitefrac<G, NC>(c, m, on, ns, n, myid, hd, x, slot, Td, spimd, yd);
         // interface type of every active pair, row gk of the group's table
         {
             int row[G];
@@ -563,7 +571,7 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
         }
         wave_sync();
 
-        R pfrhs = R(0);
+        double pfrhs = 0.0;
         bool act = false;
         bool unsure = false;
         if (n > 1) {
@@ -578,7 +586,7 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
                     const R phj = gsh<G>(phi, gj), lapj = gsh<G>(plap, gj);
                     const int idj = gsh<G>(myid, gj);
                     const int it = pt[gk * G + gj];
-                    const R e = pair_e<R>(c, itab, it, T);
+                    const R e = pair_e<R>(c, itab, it, Td);
                     const R w = (R)itab[3 * it + 1];
                     R lapaj = R(0), lapak = R(0);
                     if constexpr (MIXED) { lapaj = gsh<G>(lapabs, gj); lapak = gsh<G>(lapabs, gk); }
@@ -599,9 +607,9 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
                         R tj = R(0);
                         const R r1 = e / w;
                         tj = tj > r1 ? tj : r1;
-                        const R r2 = pair_e<R>(c, itab, it2, T) / (R)itab[3 * it2 + 1];
+                        const R r2 = pair_e<R>(c, itab, it2, Td) / (R)itab[3 * it2 + 1];
                         tj = tj > r2 ? tj : r2;
-                        const R r3 = pair_e<R>(c, itab, it3, T) / (R)itab[3 * it3 + 1];
+                        const R r3 = pair_e<R>(c, itab, it3, Td) / (R)itab[3 * it3 + 1];
                         tj = tj > r3 ? tj : r3;
                         tj *= (R)c.junctionpenalty;
                         R t3 = R(0);
@@ -621,7 +629,7 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
                 }
             }
             // chemical driving force (src/cipher.c:499-522; Chemenergy, src/material.c:480-490), FP64
-            R chs = R(0);
+            double chs = 0.0;
             if (on) {
                 double ch = d_chemenergy<NC, double>(c, m, yd, Td);
                 ch /= m.molarvolume;
@@ -631,22 +639,22 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
 #pragma unroll
                     for (int k = 0; k < ND; k++) ch -= x[c.dpo + k] * m.stoichiometry[s] * yd[s * NC + k];
                 }
-                chs = (R)ch;
+                chs = ch;
             }
             if constexpr (NC > 1) {
                 if (c.has_potential) {
-                    R wdp[NDV];
+                    double wdp[NDV];
 #pragma unroll
-                    for (int k = 0; k < ND; k++) wdp[k] = R(0);
+                    for (int k = 0; k < ND; k++) wdp[k] = 0.0;
                     #pragma unroll 1
                     for (int g = 0; g < n; g++) {
                         const int nsg = gsh<G>(ns, g);
 #pragma unroll
                         for (int s = 0; s < 2; s++) {
-                            const R st = (s < ns) ? (R)m.stoichiometry[s] : R(0);
+                            const double st = (s < ns) ? m.stoichiometry[s] : 0.0;
 #pragma unroll
                             for (int k = 0; k < ND; k++) {
-                                const R v = gsh<G>(h * st * y[s * NC + k], g);
+                                const double v = gsh<G>(hd * st * yd[s * NC + k], g);
                                 if (s < nsg) wdp[k] += v;
                             }
                         }
@@ -654,16 +662,16 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
                     #pragma unroll 1
                     for (int gk = 0; gk < n; gk++) {
                         const int idk = gsh<G>(myid, gk);
-                        const R hk = gsh<G>(h, gk);
+                        const double hk = gsh<G>(hd, gk);
                         #pragma unroll 1
                         for (int gj = gk + 1; gj < n; gj++) {
                             const int idj = gsh<G>(myid, gj);
-                            const R hj = gsh<G>(h, gj);
+                            const double hj = gsh<G>(hd, gj);
                             const cipher_interface &f = c.itf[c.ilist[idk * c.npf + idj]];
                             if (f.has_potential) {
-                                R rv = R(0);
+                                double rv = 0.0;
 #pragma unroll
-                                for (int k = 0; k < ND; k++) rv += (R)f.potential[k] * wdp[k];
+                                for (int k = 0; k < ND; k++) rv += f.potential[k] * wdp[k];
                                 if (slot == gk) chs += rv * hj;
                                 if (slot == gj) chs += rv * hk;
                             }
@@ -676,40 +684,45 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
                 // magnitude of the inputs of the interpolant-derivative product
                 R mx = R(0);
                 #pragma unroll 1
-                for (int g = 0; g < n; g++) mx = fmaxf(mx, fabsf(gsh<G>(chs, g)));
+                for (int g = 0; g < n; g++) mx = fmaxf(mx, (float)fabs(gsh<G>(chs, g)));
                 chsabs = R(12) * mx;
             }
-            chs = interp_deriv<G, R>(c, n, phi, hh, chs);
-            // unconstrained rate and active flags (src/cipher.c:524-539)
-            R ru = R(0), ruabs = R(0);
+            chs = interp_deriv<G, double>(c, n, phid, hhd, chs);
+            // unconstrained rate and active flags (src/cipher.c:524-539); the
+            // pair rates are summed in FP64 in both modes
+            double ru = 0.0;
+            R ruabs = R(0);
             R nact = R(0);
             #pragma unroll 1
             for (int gk = 0; gk < n; gk++) {
-                const R capk = gsh<G>(cap, gk), chk = gsh<G>(chs, gk), hk = gsh<G>(h, gk);
+                const R capk = gsh<G>(cap, gk);
+                const double chk = gsh<G>(chs, gk), hk = gsh<G>(hd, gk);
                 const int idk = gsh<G>(myid, gk);
                 R capak = R(0);
                 if constexpr (MIXED) capak = gsh<G>(capabs, gk);
                 #pragma unroll 1
                 for (int gj = gk + 1; gj < n; gj++) {
-                    const R capj = gsh<G>(cap, gj), chj = gsh<G>(chs, gj), hj = gsh<G>(h, gj);
+                    const R capj = gsh<G>(cap, gj);
+                    const double chj = gsh<G>(chs, gj), hj = gsh<G>(hd, gj);
                     const int idj = gsh<G>(myid, gj);
-                    const R mkj = pair_m<R>(c, itab, pt[gk * G + gj], T);
-                    const R rv = mkj * ((capk - capj) - (chk - chj) * R(8) * sqrt(hk * hj) / (R)CPI);
+                    const R mkj = pair_m<R>(c, itab, pt[gk * G + gj], Td);
+                    const double rv = (double)mkj * ((double)(capk - capj) - (chk - chj) * 8.0 * sqrt(hk * hj) / CPI);
                     if (slot == gk) ru += rv;
                     if (slot == gj) ru -= rv;
                     if constexpr (MIXED) {
                         const R capaj = gsh<G>(capabs, gj);
                         if (slot == gk || slot == gj)
                             ruabs += fabsf(mkj) * (fabsf(capk) + fabsf(capj) + capak + capaj
-                                                   + (fabsf(chk) + fabsf(chj) + R(2) * chsabs) * R(8) * sqrtf(hk * hj) / (R)CPI);
+                                                   + ((float)fabs(chk) + (float)fabs(chj) + R(2) * chsabs) * R(8)
+                                                         * (float)sqrt(hk * hj) / (R)CPI);
                     }
                 }
                 if (slot == gk) {
-                    act = (phid > CIPHER_TOL && phid < 1.0 - CIPHER_TOL) || (phid < CIPHER_TOL && ru > R(0))
-                          || (phid > 1.0 - CIPHER_TOL && ru < R(0));
+                    act = (phid > CIPHER_TOL && phid < 1.0 - CIPHER_TOL) || (phid < CIPHER_TOL && ru > 0.0)
+                          || (phid > 1.0 - CIPHER_TOL && ru < 0.0);
                     // the decision reads the sign of ru: is it inside the FP32 band?
                     if constexpr (MIXED)
-                        unsure = !(phid > CIPHER_TOL && phid < 1.0 - CIPHER_TOL) && fabsf(ru) <= 1.0e-4f * ruabs;
+                        unsure = !(phid > CIPHER_TOL && phid < 1.0 - CIPHER_TOL) && (float)fabs(ru) <= 1.0e-4f * ruabs;
                 }
             }
             #pragma unroll 1
@@ -719,20 +732,22 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
             for (int gk = 0; gk < n; gk++) {
                 const int ak = gsh<G>((int)act, gk);
                 if (!ak) continue;
-                const R capk = gsh<G>(cap, gk), chk = gsh<G>(chs, gk), hk = gsh<G>(h, gk);
+                const R capk = gsh<G>(cap, gk);
+                const double chk = gsh<G>(chs, gk), hk = gsh<G>(hd, gk);
                 const int idk = gsh<G>(myid, gk);
                 #pragma unroll 1
                 for (int gj = gk + 1; gj < n; gj++) {
                     const int aj = gsh<G>((int)act, gj);
-                    const R capj = gsh<G>(cap, gj), chj = gsh<G>(chs, gj), hj = gsh<G>(h, gj);
+                    const R capj = gsh<G>(cap, gj);
+                    const double chj = gsh<G>(chs, gj), hj = gsh<G>(hd, gj);
                     const int idj = gsh<G>(myid, gj);
                     if (!aj) continue;
-                    const R mkj = pair_m<R>(c, itab, pt[gk * G + gj], T);
-                    const R rv = mkj * ((capk - capj) - (chk - chj) * R(8) * sqrt(hk * hj) / (R)CPI);
+                    const R mkj = pair_m<R>(c, itab, pt[gk * G + gj], Td);
+                    const double rv = (double)mkj * ((double)(capk - capj) - (chk - chj) * 8.0 * sqrt(hk * hj) / CPI);
                     if (slot == gk) pfrhs += rv;
                     if (slot == gj) pfrhs -= rv;
                 }
-                if (slot == gk) pfrhs /= nact;
+                if (slot == gk) pfrhs /= (double)nact;
             }
         }
         const int gbase = (threadIdx.x & 63) & ~(G - 1);
@@ -755,8 +770,8 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
         R tmrhs = tlap.s;
         if (c.has_theat || c.has_trate) {
             R cpv = R(0);
-            const R cpt = (on && c.ts[m.specific_heat].n) ? h * dts<R>(c, m.specific_heat, T) : R(0);
-            const R lht = (on && c.ts[m.latent_heat].n) ? pfrhs * dts<R>(c, m.latent_heat, T) : R(0);
+            const R cpt = (on && c.ts[m.specific_heat].n) ? h * (R)dts<double>(c, m.specific_heat, Td) : R(0);
+            const R lht = (on && c.ts[m.latent_heat].n) ? (R)pfrhs * (R)dts<double>(c, m.latent_heat, Td) : R(0);
             #pragma unroll 1
             for (int g = 0; g < n; g++) {
                 const int idg = gsh<G>(myid, g);

@@ -310,3 +310,19 @@ def cpu_solve(wl, X, t0, t_end, dt, params, max_steps=0):
             rejected += 1
         dt = dt_next
     return X, dict(steps=steps, rejected=rejected, t=t, dt=dt, history=dts)
+
+
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.abspath(__file__)), "golden")
+REF_CASES = ["bc_chem", "bc_thermal", "gb_precipitate", "grain_growth", "nucleation_7050", "nucleation_crti",
+             "segregation", "sl2_t250", "solidification"]
+
+
+def ref_case(name):
+    """A fixture derived from one of the reference's example configurations
+    (tests/golden/make_reference_cases.py)."""
+    import os
+    key = ("ref", name)
+    if key not in _cache:
+        with open(os.path.join(GOLDEN, f"ref_{name}.yaml")) as f:
+            _cache[key] = W.Workload(f.read(), name)
+    return _cache[key]

@@ -53,6 +53,9 @@ WORKLOADS = {
     # and chemical/thermal Neumann and Dirichlet boundaries
     "chem4_thermal": lambda: H.chem(thermal=True),
 }
+# the reference's example configurations (tests/golden, reduced grids)
+for _n in H.REF_CASES:
+    WORKLOADS[f"ref_{_n}"] = (lambda n=_n: H.ref_case(n))
 
 
 def _ulp_spread(wl, X, F, trials=3):
@@ -198,3 +201,18 @@ def test_rhs_fp32_mixed_within_1e6(name):
     for f, e in errs.items():
         assert e <= RTOL32, f"{name} field {f}: FP32-mixed rel err {e:.3e}"
     assert np.array_equal(_host(a32), _host(a64))
+
+
+def test_gradient_calculation_is_reported_unsupported():
+    """Anisotropy with LSQ gradients (the reference's Anisotropy example) is not
+    on the GPU path: context creation or the RHS says so."""
+    import os
+    from cipher_amd import workload as W
+    with open(os.path.join(H.GOLDEN, "ref_anisotropy.yaml")) as f:
+        wl = W.Workload(f.read(), "anisotropy")
+    with pytest.raises(abi.CipherError) as e:
+        ctx = _ctx(wl)
+        import torch
+        X = torch.zeros(wl.ncells * wl.layout["ndof"], dtype=torch.float64, device=ctx.device)
+        ctx.rhs(X)
+    assert e.value.code == 5

@@ -1,0 +1,78 @@
+"""The reference's own example configurations (material databases, interface
+tables, boundary conditions; reduced grids in tests/golden) through the
+oracle: they load, and the invariants that pin the oracle hold on them."""
+import numpy as np
+import pytest
+
+import oracle
+from cipher_amd import abi
+from tests import helpers as H
+from tests.test_oracle_pins import _dcdt
+
+
+@pytest.mark.parametrize("name", H.REF_CASES)
+def test_reference_case_invariants(name):
+    wl = H.ref_case(name)
+    cfg, L = wl.cfg, wl.layout
+    X = H.perturb(wl, H.evolve(wl, H.initial_state(wl), steps=3), seed=5)
+    F = oracle.rhs(cfg, wl.mesh, X)
+    assert np.isfinite(F).all()
+    r = F.reshape(-1, L["ndof"])[:, L["pf_offset"]:L["pf_offset"] + L["pf_size"]]
+    assert (np.abs(r.sum(axis=1)) <= 1e-12 * (np.abs(r).max(axis=1) + 1e-300)).all()
+
+
+@pytest.mark.parametrize("name", [n for n in H.REF_CASES if n not in ("grain_growth",)])
+def test_reference_case_conserves_composition(name):
+    """Uniform chemical potential, no boundary flux or sources: the
+    chemical-potential rate cancels every other composition change per cell
+    (cells with a quadratic phase under a temperature rate excepted, see the
+    oracle pins)."""
+    wl = H.ref_case(name)
+    cfg, L = wl.cfg, wl.layout
+    nd = cfg.ncp - 1
+    X = H.perturb(wl, H.initial_state(wl), seed=7).reshape(-1, L["ndof"])
+    dpo = L["dp_offset"]
+    X[:, dpo:dpo + nd] = X[0, dpo:dpo + nd]
+    saved = (cfg.struct.nbcs, cfg.struct.nsource)
+    cfg.struct.nbcs = 0
+    quad = [i for i, m in enumerate(cfg.materials) if m.model == abi.CHEM_QUADRATIC]
+    # the reference gives quadratic phases a zero EX tangent
+    # (src/material.c:957-959) although their site fractions see EX through
+    # the site potential: conservation holds for their consistent EX = 0
+    # (their explicit potential is zero, so the EX rate keeps it there)
+    SP = L["sp_size"]
+    for cell in range(wl.ncells):
+        n = int(round(X[cell, 0]))
+        for g in range(n):
+            if cfg.phase_material[int(round(X[cell, 1 + g]))] in quad:
+                X[cell, L["ex_offset"] + g * SP:L["ex_offset"] + (g + 1) * SP] = 0.0
+    try:
+        F = oracle.rhs(cfg, wl.mesh, X.ravel()).reshape(X.shape)
+    finally:
+        cfg.struct.nbcs = saved[0]
+    has_src = any(m.nsources for m in cfg.materials) or any(i.nsources for i in cfg.interfaces)
+    if has_src:
+        pytest.skip("the configuration has sources")
+    dcs, dfixs = [], []
+    for cell in range(wl.ncells):
+        n = int(round(X[cell, 0]))
+        mats = [cfg.phase_material[int(round(v))] for v in X[cell, 1:1 + n]]
+        if cfg.struct.has_temperature_rate and any(m in quad for m in mats):
+            continue
+        dcs.append(np.abs(_dcdt(wl, X[cell], F[cell])).max())
+        dfixs.append(np.abs(_dcdt(wl, X[cell], F[cell], mu_fixed=True)).max())
+    dcs, dfixs = np.array(dcs), np.array(dfixs)
+    assert dfixs.max() > 0
+    # per cell, against the cell's own change, with a floor at the rounding
+    # level of the largest change in the mesh
+    assert (dcs <= 1e-5 * dfixs + 1e-8 * dfixs.max()).all(), (dcs / np.maximum(dfixs, 1e-300)).max()
+
+
+def test_anisotropy_case_is_reported_unsupported():
+    import os
+    from cipher_amd import workload as W
+    with open(os.path.join(H.GOLDEN, "ref_anisotropy.yaml")) as f:
+        wl = W.Workload(f.read(), "anisotropy")
+    assert wl.cfg.gradient_calculation == 1
+    with pytest.raises(RuntimeError, match="5"):
+        oracle.rhs(wl.cfg, wl.mesh, H.initial_state(wl))
