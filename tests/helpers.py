@@ -326,3 +326,27 @@ def ref_case(name):
         with open(os.path.join(GOLDEN, f"ref_{name}.yaml")) as f:
             _cache[key] = W.Workload(f.read(), name)
     return _cache[key]
+
+
+def crowded(npf=24, cap=16, nmin=6, nmax=16, res=(16, 16), seed=3):
+    """A synthetic state with large active sets (n up to the capacity) on a
+    2D periodic grid: every cell holds a random sorted subset of the phases
+    with random fractions on the simplex; it exercises the widest lane groups
+    and long neighbour-list searches."""
+    key = ("crowded", npf, cap, nmin, nmax, res, seed)
+    if key not in _cache:
+        txt = W.grain_growth_yaml(res=res, ngrains=npf, periodic=True, capacity=cap, energy_range=(0.5, 1.5),
+                                  ntypes=12)
+        wl = W.Workload(txt, f"crowded{cap}")
+        rng = np.random.default_rng(seed)
+        L = wl.layout
+        X = np.zeros((wl.ncells, L["ndof"]))
+        for c in range(wl.ncells):
+            n = int(rng.integers(nmin, min(nmax, cap) + 1))
+            ids = np.sort(rng.choice(npf, size=n, replace=False))
+            X[c, 0] = n
+            X[c, 1:1 + n] = ids
+            X[c, L["pf_offset"]:L["pf_offset"] + n] = rng.dirichlet(np.ones(n))
+        X[:, L["tm_offset"]] = 1.0
+        _cache[key] = (wl, X.ravel())
+    return _cache[key]
