@@ -99,7 +99,7 @@ class Mesh(C.Structure):
     _fields_ = [("face_cells", C.POINTER(C.c_int32)), ("cellgeom", C.POINTER(C.c_double)),
                 ("bface_cell", C.POINTER(C.c_int32)), ("bface_id", C.POINTER(C.c_int32)),
                 ("ncells", C.c_int32), ("nfaces", C.c_int32), ("nbfaces", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("pad_", C.c_int32), ("centroid", C.POINTER(C.c_double)), ("period", C.c_double * 3)]
 
 
 class TsParams(C.Structure):

@@ -33,6 +33,8 @@ int cipher_fail(int code, const std::string &msg) { return fail(code, msg); }
     } while (0)
 
 static void free_mesh(cipher_ctx *c) {
+    if (c->dc.gw) (void)hipFree((void *)c->dc.gw);
+    c->dc.gw = nullptr;
     void *ptrs[] = {c->cell_off, c->adj, c->bcell_off, c->bface_id, c->cellgeom, c->cm, c->glist, c->list, c->bins};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -90,8 +92,13 @@ static int validate(const cipher_config *cfg) {
     if (cfg->npf > 65535) return fail(CIPHER_ERR_ARG, "phase ids are uint16 (npf <= 65535)");
     if (cfg->nf < 1 || !cfg->interface) return fail(CIPHER_ERR_ARG, "no interfaces");
     if (!cfg->phase_material || !cfg->interface_list) return fail(CIPHER_ERR_ARG, "missing mappings");
-    if (cfg->gradient_calculation)
-        return fail(CIPHER_ERR_UNSUPPORTED, "gradient_calculation (anisotropy) is not on the GPU path");
+    for (int i = 0; i < cfg->nf; i++) {
+        const cipher_interface &f = cfg->interface[i];
+        const int ne = f.e_naniso > 0 ? f.e_aniso + (cfg->dim + 1) * f.e_naniso : 0;
+        const int nm = f.m_naniso > 0 ? f.m_aniso + (cfg->dim + 1) * f.m_naniso : 0;
+        if (f.e_naniso < 0 || f.m_naniso < 0 || ne > cfg->naniso || nm > cfg->naniso || ((ne || nm) && !cfg->aniso))
+            return fail(CIPHER_ERR_ARG, "anisotropy table out of range");
+    }
     if (cfg->interpolation < CIPHER_INTERP_LINEAR || cfg->interpolation > CIPHER_INTERP_CUBIC)
         return fail(CIPHER_ERR_ARG, "interpolation");
     if (cfg->ntseries < 1 || !cfg->tseries) return fail(CIPHER_ERR_ARG, "tseries pool must hold the empty series");
@@ -190,6 +197,8 @@ extern "C" int cipher_ctx_create(const cipher_config *cfg, int precision, cipher
     d.SP = c->L.sp_size;
     d.SF = c->L.sf_size;
     d.has_tcond = d.has_theat = d.has_sources = d.has_potential = d.has_imobc = d.has_iact = 0;
+    d.has_grad = cfg->gradient_calculation ? 1 : 0;
+    d.gw = nullptr;
     for (int m = 0; m < cfg->nmat; m++) {
         const cipher_material &mt = cfg->material[m];
         if (cfg->tseries[mt.tconductivity].n) d.has_tcond = 1;
@@ -291,6 +300,55 @@ extern "C" int cipher_set_temperature_rate(cipher_ctx *c, int has_rate, double r
     return CIPHER_OK;
 }
 
+template <int DIM>
+static void lsq_cell(const cipher_mesh *m, const std::vector<int32_t> &off, const std::vector<int32_t> &adj,
+                     long long cell, double *gw) {
+    const int k0 = off[cell], nls = off[cell + 1] - off[cell];
+    double a[24 * DIM], ata[DIM * DIM], atainv[DIM * DIM], gm[24 * DIM];
+    for (int j = 0; j < nls; j++) {
+        const long long nb = adj[k0 + j] >> 1;
+        for (int d = 0; d < DIM; d++) {
+            double v = m->centroid[nb * DIM + d];
+            v -= m->centroid[cell * DIM + d];
+            const double L = m->period[d];
+            if (L > 0.0) {
+                if (v > 0.5 * L) v -= L;
+                else if (v < -0.5 * L) v += L;
+            }
+            a[j * DIM + d] = v;
+        }
+    }
+    for (int di = 0; di < DIM; di++)
+        for (int dj = di; dj < DIM; dj++) {
+            double val = 0.0;
+            for (int j = 0; j < nls; j++) val += a[j * DIM + di] * a[j * DIM + dj];
+            ata[di * DIM + dj] = val;
+            ata[dj * DIM + di] = val;
+        }
+    dinvert<DIM, double>(atainv, ata);
+    for (int q = 0; q < 24 * DIM; q++) gm[q] = 0.0;
+    for (int di = 0; di < DIM; di++)
+        for (int j = 0; j < nls; j++)
+            for (int d = 0; d < DIM; d++) gm[di * nls + j] += atainv[di * DIM + d] * a[j * DIM + d];
+    for (int j = 0; j < nls; j++)
+        for (int d = 0; d < DIM; d++) gw[(size_t)(k0 + j) * DIM + d] = gm[d * nls + j];
+}
+
+static int lsq_weights(int dim, const cipher_mesh *m, const std::vector<int32_t> &off,
+                       const std::vector<int32_t> &adj, std::vector<double> &gw) {
+    if (!m->centroid) return fail(CIPHER_ERR_ARG, "gradient_calculation needs cell centroids");
+    const long long N = m->ncells;
+    gw.assign((size_t)std::max<long long>(off[N], 1) * dim, 0.0);
+    for (long long cell = 0; cell < N; cell++) {
+        if (off[cell + 1] - off[cell] > 24)
+            return fail(CIPHER_ERR_UNSUPPORTED, "more than 24 face neighbours (src/cipher.c:1240)");
+        if (dim == 1) lsq_cell<1>(m, off, adj, cell, gw.data());
+        else if (dim == 2) lsq_cell<2>(m, off, adj, cell, gw.data());
+        else lsq_cell<3>(m, off, adj, cell, gw.data());
+    }
+    return CIPHER_OK;
+}
+
 extern "C" int cipher_upload_mesh(cipher_ctx *c, const cipher_mesh *m) {
     if (!c || !m) return fail(CIPHER_ERR_ARG, "null argument");
     if (m->ncells < 0 || m->nfaces < 0 || m->nbfaces < 0) return fail(CIPHER_ERR_ARG, "negative sizes");
@@ -340,6 +398,20 @@ extern "C" int cipher_upload_mesh(cipher_ctx *c, const cipher_mesh *m) {
     HIPCHK(hipMemcpy(c->bcell_off, boff.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->bface_id, bid.data(), sizeof(int32_t) * bid.size(), hipMemcpyHostToDevice));
     if (N) HIPCHK(hipMemcpy(c->cellgeom, m->cellgeom, sizeof(double) * N, hipMemcpyHostToDevice));
+    if (c->dc.has_grad) {
+        // least-squares gradient weights (A^T A)^-1 A^T per cell over its face
+        // neighbours in face order, src/cipher.c:1233-1282
+        std::vector<double> gw;
+        int rc = lsq_weights(c->dc.dim, m, off, adj, gw);
+        if (rc) {
+            free_mesh(c);
+            return rc;
+        }
+        double *dgw = nullptr;
+        HIPCHK(hipMalloc(&dgw, sizeof(double) * gw.size()));
+        HIPCHK(hipMemcpy(dgw, gw.data(), sizeof(double) * gw.size(), hipMemcpyHostToDevice));
+        c->dc.gw = dgw;
+    }
     c->ncells = N;
     return CIPHER_OK;
 }

@@ -28,6 +28,8 @@ struct DevCfg {
     const int32_t *phase_material;
     const int32_t *ilist;
     const double *itab;       // [e0, width, m0] per interface type
+    int has_grad;             // gradient_calculation: anisotropic interfaces
+    const double *gw;         // least-squares gradient weights, dim per face-list entry
     double junctionpenalty;
     double trate;
     int has_trate;
@@ -122,7 +124,7 @@ template <int N> struct Cnt { static constexpr int v = N > 0 ? N : 1; };
 // Invertmatrix for n <= 4 (src/utility.c:232-405): adjugate of a matrix taken
 // to be symmetric, with the reference's cofactor term order.
 template <int ND, class R>
-__device__ inline void dinvert(R *d, const R *s) {
+__host__ __device__ inline void dinvert(R *d, const R *s) {
     R det;
     if constexpr (ND == 1) {
         d[0] = R(1) / s[0];

@@ -343,6 +343,15 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
         const bool need_lap = n > 1;
         const int kb = a.cell_off[cell];
         const int ke = (need_lap || NC > 1 || c.has_tcond) ? a.cell_off[cell + 1] : kb;
+        // least-squares gradient of the slot's phase fraction
+        // (src/cipher.c:103-131): minus the weighted own value over every
+        // neighbour first, then the common neighbours' values in face order
+        const bool grad_on = c.has_grad && need_lap;
+        double gr[3] = {0.0, 0.0, 0.0};
+        if (grad_on) {
+            for (int k = kb; k < ke; k++)
+                for (int d = 0; d < c.dim; d++) gr[d] -= c.gw[(long long)k * c.dim + d] * phid;
+        }
         for (int f0 = kb; f0 < ke; f0 += G) {
             const int nfc = min(G, ke - f0);
             // -- face-parallel part
@@ -362,7 +371,8 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
             const int nbn = clamp_count(nbnd, c.P);
             const int Ln = selfL ? n : nbn;
             const int Lid0 = selfL ? id0 : clamp_id(id0d, c.npf);
-            const int guard = (fon && need_lap && (Ln > 1 || Lid0 > 1)) ? 1 : 0;
+            const int lguard = (fon && need_lap && (Ln > 1 || Lid0 > 1)) ? 1 : 0;
+            const int guard = (lguard || (fon && grad_on)) ? 1 : 0;
             R cterm[NDV];
 #pragma unroll
             for (int q = 0; q < NDV; q++) cterm[q] = R(0);
@@ -470,16 +480,21 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
                         }
                     }
                     R pv = R(0);
+                    double pvd = 0.0;
 #pragma unroll
                     for (int r = 0; r < RM; r++) {
                         if (r * G < nbnt) {
-                            const R cand = gsh<G>((R)phc[r], pos < 0 ? 0 : (pos & (G - 1)));
-                            if (pos >= 0 && pos / G == r) pv = cand;
+                            const double cand = gsh<G>(phc[r], pos < 0 ? 0 : (pos & (G - 1)));
+                            if (pos >= 0 && pos / G == r) pvd = cand;
                         }
                     }
+                    pv = (R)pvd;
+                    if (grad_on && on && pos >= 0)
+                        for (int d = 0; d < c.dim; d++) gr[d] += c.gw[(long long)(f0 + t) * c.dim + d] * pvd;
                     const R fft = gsh<G>(ff, t), volt = gsh<G>(vol, t);
                     const int sL = gsh<G>((int)selfL, t);
-                    if (on && pos >= 0) {
+                    const int lgt = gsh<G>(lguard, t);     // every lane of the group shuffles
+                    if (on && pos >= 0 && lgt) {
                         R term;
                         if (sL) {
                             const R fl = pv - phi;
@@ -569,6 +584,47 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
             for (int gj = 0; gj < G; gj++)
                 if (on && gj > slot && gj < n) pt[slot * G + gj] = row[gj];
         }
+        // anisotropic interface energy and mobility of every active pair
+        // (src/cipher.c:429-469): 1 + sum_dir (n.dir)^4 val along the unit
+        // difference of the two gradients, then the constant and the
+        // activation factor; row gk of the group's tables
+        double *pe_tab = (double *)(ptype + CPB * G * G) + lc * G * G * 2;
+        double *pm_tab = pe_tab + G * G;
+        if (grad_on) {
+            #pragma unroll 1
+            for (int gj = 0; gj < n; gj++) {
+                double gjv[3];
+                for (int d = 0; d < c.dim; d++) gjv[d] = gsh<G>(gr[d], gj);
+                if (on && gj > slot) {
+                    const cipher_interface &f = c.itf[pt[slot * G + gj]];
+                    double nv[3], rn = 0.0;
+                    for (int d = 0; d < c.dim; d++) {
+                        nv[d] = gr[d] - gjv[d];
+                        rn += nv[d] * nv[d];
+                    }
+                    rn = sqrt(rn);
+                    if (rn > 1e-16)
+                        for (int d = 0; d < c.dim; d++) nv[d] /= rn;
+                    double ev = 1.0, mv = 1.0;
+                    for (int r = 0; r < f.e_naniso; r++) {
+                        double dot = 0.0;
+                        for (int d = 0; d < c.dim; d++) dot += nv[d] * c.aniso[f.e_aniso + c.dim * r + d];
+                        ev += dfastpow(dot, 4u) * c.aniso[f.e_aniso + c.dim * f.e_naniso + r];
+                    }
+                    for (int r = 0; r < f.m_naniso; r++) {
+                        double dot = 0.0;
+                        for (int d = 0; d < c.dim; d++) dot += nv[d] * c.aniso[f.m_aniso + c.dim * r + d];
+                        mv += dfastpow(dot, 4u) * c.aniso[f.m_aniso + c.dim * f.m_naniso + r];
+                    }
+                    ev *= f.e0;
+                    if (c.has_iact && c.ts[f.e_act].n) ev *= dexp<double>(dts<double>(c, f.e_act, Td) / CRG / Td);
+                    mv *= f.m0;
+                    if (c.has_iact && c.ts[f.m_act].n) mv *= dexp<double>(dts<double>(c, f.m_act, Td) / CRG / Td);
+                    pe_tab[slot * G + gj] = ev;
+                    pm_tab[slot * G + gj] = mv;
+                }
+            }
+        }
         wave_sync();
 
         double pfrhs = 0.0;
@@ -586,7 +642,7 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
                     const R phj = gsh<G>(phi, gj), lapj = gsh<G>(plap, gj);
                     const int idj = gsh<G>(myid, gj);
                     const int it = pt[gk * G + gj];
-                    const R e = pair_e<R>(c, itab, it, Td);
+                    const R e = grad_on ? (R)pe_tab[gk * G + gj] : pair_e<R>(c, itab, it, Td);
                     const R w = (R)itab[3 * it + 1];
                     R lapaj = R(0), lapak = R(0);
                     if constexpr (MIXED) { lapaj = gsh<G>(lapabs, gj); lapak = gsh<G>(lapabs, gk); }
@@ -607,9 +663,9 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
                         R tj = R(0);
                         const R r1 = e / w;
                         tj = tj > r1 ? tj : r1;
-                        const R r2 = pair_e<R>(c, itab, it2, Td) / (R)itab[3 * it2 + 1];
+                        const R r2 = (grad_on ? (R)pe_tab[gk * G + gi] : pair_e<R>(c, itab, it2, Td)) / (R)itab[3 * it2 + 1];
                         tj = tj > r2 ? tj : r2;
-                        const R r3 = pair_e<R>(c, itab, it3, Td) / (R)itab[3 * it3 + 1];
+                        const R r3 = (grad_on ? (R)pe_tab[gj * G + gi] : pair_e<R>(c, itab, it3, Td)) / (R)itab[3 * it3 + 1];
                         tj = tj > r3 ? tj : r3;
                         tj *= (R)c.junctionpenalty;
                         R t3 = R(0);
@@ -705,7 +761,7 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
                     const R capj = gsh<G>(cap, gj);
                     const double chj = gsh<G>(chs, gj), hj = gsh<G>(hd, gj);
                     const int idj = gsh<G>(myid, gj);
-                    const R mkj = pair_m<R>(c, itab, pt[gk * G + gj], Td);
+                    const R mkj = grad_on ? (R)pm_tab[gk * G + gj] : pair_m<R>(c, itab, pt[gk * G + gj], Td);
                     const double rv = (double)mkj * ((double)(capk - capj) - (chk - chj) * 8.0 * sqrt(hk * hj) / CPI);
                     if (slot == gk) ru += rv;
                     if (slot == gj) ru -= rv;
@@ -742,7 +798,7 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
                     const double chj = gsh<G>(chs, gj), hj = gsh<G>(hd, gj);
                     const int idj = gsh<G>(myid, gj);
                     if (!aj) continue;
-                    const R mkj = pair_m<R>(c, itab, pt[gk * G + gj], Td);
+                    const R mkj = grad_on ? (R)pm_tab[gk * G + gj] : pair_m<R>(c, itab, pt[gk * G + gj], Td);
                     const double rv = (double)mkj * ((double)(capk - capj) - (chk - chj) * 8.0 * sqrt(hk * hj) / CPI);
                     if (slot == gk) pfrhs += rv;
                     if (slot == gj) pfrhs -= rv;
@@ -1029,8 +1085,8 @@ __global__ void k_fill_i32(int32_t *p, int n, int32_t v) {
 
 // LDS bytes of a k_rhs block of cpb cells
 static inline size_t rhs_lds_bytes(const DevCfg &c, int G, int cpb) {
-    (void)c;
-    return (size_t)cpb * G * G * sizeof(int);
+    // pair interface types; with gradients also the pair energies/mobilities
+    return (size_t)cpb * G * G * (sizeof(int) + (c.has_grad ? 2 * sizeof(double) : 0));
 }
 
 // One k_rhs launch over a device-counted cell list: blocks of 256 threads in

@@ -152,19 +152,28 @@ class Forest:
         else:
             bc = np.zeros(0, dtype=np.int64)
             bi = np.zeros(0, dtype=np.int64)
+        period = [float(self.size[d]) if self.periodic[d] else 0.0 for d in range(dim)]
         return FlatMesh(F.astype(np.int32), self.cellgeom(), bc.astype(np.int32), bi.astype(np.int32),
-                        dim=self.dim)
+                        dim=self.dim, centroid=self.centroids(), period=period)
+
+    def centroids(self):
+        """Cell centres in physical units, (ncells, dim)."""
+        dx = self.size / self.res
+        ext = self.cell_extent().astype(np.float64)
+        return (self.anchor.astype(np.float64) + 0.5 * ext[:, None]) * dx[None, :]
 
 
 class FlatMesh:
     """Flattened mesh arrays in the layout of cipher_mesh."""
 
-    def __init__(self, faces, cellgeom, bface_cell, bface_id, dim):
+    def __init__(self, faces, cellgeom, bface_cell, bface_id, dim, centroid=None, period=None):
         self.faces = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 2)
         self.cellgeom = np.ascontiguousarray(cellgeom, dtype=np.float64)
         self.bface_cell = np.ascontiguousarray(bface_cell, dtype=np.int32)
         self.bface_id = np.ascontiguousarray(bface_id, dtype=np.int32)
         self.dim = dim
+        self.centroid = None if centroid is None else np.ascontiguousarray(centroid, dtype=np.float64).reshape(-1)
+        self.period = list(period or []) + [0.0] * (3 - len(period or []))
 
     @property
     def ncells(self):
@@ -183,6 +192,10 @@ class FlatMesh:
         m.bface_cell = self.bface_cell.ctypes.data_as(C.POINTER(C.c_int32))
         m.bface_id = self.bface_id.ctypes.data_as(C.POINTER(C.c_int32))
         m.ncells, m.nfaces, m.nbfaces = self.ncells, self.nfaces, int(self.bface_cell.size)
+        if self.centroid is not None:
+            m.centroid = self.centroid.ctypes.data_as(C.POINTER(C.c_double))
+        for d in range(3):
+            m.period[d] = self.period[d]
         m._keep = self
         return m
 

@@ -201,6 +201,12 @@ typedef struct cipher_mesh {
     int32_t nfaces;
     int32_t nbfaces;
     int32_t pad_;
+    /* Cell centroids, ncells*dim (x fastest per cell), for the least-squares
+     * gradients of gradient_calculation (src/cipher.c:1233-1282); may be NULL
+     * otherwise.  period[d] > 0: the domain length of a periodic direction,
+     * a neighbour's offset is taken as the nearest periodic image. */
+    const double *centroid;
+    double period[3];
 } cipher_mesh;
 
 typedef struct cipher_ctx cipher_ctx;
@@ -219,8 +225,10 @@ int cipher_set_stream(cipher_ctx *ctx, void *stream);
 /* Update the load-case temperature rate (src/cipher.c:568). */
 int cipher_set_temperature_rate(cipher_ctx *ctx, int has_rate, double rate);
 
-/* Upload a flattened mesh; builds the per-cell face lists and preallocates
- * every scratch buffer so that later calls allocate nothing. */
+/* Upload a flattened mesh; builds the per-cell face lists (and, with
+ * gradient_calculation, each cell's least-squares gradient weights over its
+ * face neighbours) and preallocates every scratch buffer so that later calls
+ * allocate nothing. */
 int cipher_upload_mesh(cipher_ctx *ctx, const cipher_mesh *mesh);
 
 /* F = RHS(t, X): RHSFunction, src/cipher.c:27-681.  X_dev and F_dev hold

@@ -52,6 +52,7 @@ def lib():
         L.orc_chemenergy_phase.argtypes = [P(abi.Config), C.c_int, vp, C.c_double]
         L.orc_chemenergy_phase.restype = C.c_double
         L.orc_mobility_volumeref.argtypes = [P(abi.Config), vp, vp, vp]
+        L.orc_gradients.argtypes = [P(abi.Config), P(abi.Mesh), vp, vp]
         _lib = L
     return _lib
 
@@ -217,3 +218,14 @@ def mobility_volumeref(cfg, mob, comp):
     c = np.ascontiguousarray(comp, dtype=np.float64)
     lib().orc_mobility_volumeref(C.byref(cfg.struct), _p(out), _p(m), _p(c))
     return out[:nd * nd].reshape(nd, nd)
+
+
+def gradients(cfg, mesh, X):
+    """LSQ gradients of the active phase fractions: (ncells, 16, dim)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    g = np.zeros(mesh.ncells * 16 * cfg.dim)
+    ms = mesh.struct()
+    rc = lib().orc_gradients(C.byref(cfg.struct), C.byref(ms), _p(X), _p(g))
+    if rc:
+        raise RuntimeError(f"orc_gradients failed: {rc}")
+    return g.reshape(mesh.ncells, 16, cfg.dim)

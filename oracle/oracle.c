@@ -716,9 +716,130 @@ static void chemsource(const cipher_config *cfg, double *out, const double *avgc
 }
 
 /* ------------------------------------------------------------------ */
+/* face adjacency (both directions) used by the set unions              */
+/* ------------------------------------------------------------------ */
+static int build_adjacency(const cipher_mesh *mesh, int **off_out, int **nbr_out)
+{
+    int N = mesh->ncells;
+    int *off = calloc((size_t)N + 1, sizeof(int));
+    int *nbr = malloc((size_t)(2 * mesh->nfaces + 1) * sizeof(int));
+    if (!off || !nbr) { free(off); free(nbr); return -1; }
+    for (int f = 0; f < mesh->nfaces; f++) {
+        off[mesh->face_cells[2 * f] + 1]++;
+        off[mesh->face_cells[2 * f + 1] + 1]++;
+    }
+    for (int c = 0; c < N; c++) off[c + 1] += off[c];
+    int *pos = malloc((size_t)N * sizeof(int));
+    if (!pos) { free(off); free(nbr); return -1; }
+    memcpy(pos, off, (size_t)N * sizeof(int));
+    for (int f = 0; f < mesh->nfaces; f++) {
+        int a = mesh->face_cells[2 * f], b = mesh->face_cells[2 * f + 1];
+        nbr[pos[a]++] = b;
+        nbr[pos[b]++] = a;
+    }
+    free(pos);
+    *off_out = off;
+    *nbr_out = nbr;
+    return 0;
+}
+
+#define PMAX 16
+
+/* Least-squares gradient weights, src/cipher.c:1233-1282: per cell, the
+ * rows of A are the neighbours' centroid offsets (face neighbours in face
+ * order; a periodic direction takes the nearest image), and the weights are
+ * (A^T A)^-1 A^T, stored per adjacency entry: g[k*dim + d]. */
+static int lsq_setup(const cipher_config *cfg, const cipher_mesh *mesh, const int *off, const int *nbr, double **out)
+{
+    const int N = mesh->ncells, dim = cfg->dim;
+    if (!mesh->centroid) return CIPHER_ERR_ARG;
+    double *g = calloc((size_t)(off[N] > 0 ? off[N] : 1) * dim, sizeof(double));
+    if (!g) return CIPHER_ERR_ARG;
+    for (int cell = 0; cell < N; cell++) {
+        const int nls = off[cell + 1] - off[cell];
+        double a[24 * 3], ata[9], atainv[9], gm[24 * 3];
+        if (nls > 24) { free(g); return CIPHER_ERR_ARG; }
+        for (int j = 0; j < nls; j++) {
+            const int nb = nbr[off[cell] + j];
+            for (int d = 0; d < dim; d++) {
+                double v = mesh->centroid[(size_t)nb * dim + d];
+                v -= mesh->centroid[(size_t)cell * dim + d];
+                const double L = mesh->period[d];
+                if (L > 0.0) {
+                    if (v > 0.5 * L) v -= L;
+                    else if (v < -0.5 * L) v += L;
+                }
+                a[j * dim + d] = v;
+            }
+        }
+        for (int di = 0; di < dim; di++)
+            for (int dj = di; dj < dim; dj++) {
+                double val = 0.0;
+                for (int j = 0; j < nls; j++) val += a[j * dim + di] * a[j * dim + dj];
+                ata[di * dim + dj] = val;
+                ata[dj * dim + di] = val;
+            }
+        orc_invert_matrix(atainv, ata, dim);
+        memset(gm, 0, sizeof(gm));
+        for (int di = 0; di < dim; di++)
+            for (int j = 0; j < nls; j++)
+                for (int d = 0; d < dim; d++) gm[di * nls + j] += atainv[di * dim + d] * a[j * dim + d];
+        for (int j = 0; j < nls; j++)
+            for (int d = 0; d < dim; d++) g[(size_t)(off[cell] + j) * dim + d] = gm[d * nls + j];
+    }
+    *out = g;
+    return CIPHER_OK;
+}
+
+/* Phase-field gradients of every cell's active phases, src/cipher.c:82-146:
+ * grad[(cell*PMAX + g)*dim + d]. */
+static void lsq_gradients(const cipher_config *cfg, const cipher_mesh *mesh, const int *off, const int *nbr,
+                          const double *gw, const double *X, int D, int aso, int pfo, double *grad)
+{
+    const int N = mesh->ncells, dim = cfg->dim;
+    uint16_t sl[PMAX + 1], nl[PMAX + 1], is[PMAX + 1], iL[PMAX + 1], iR[PMAX + 1];
+    for (int cell = 0; cell < N; cell++) {
+        const double *x = X + (size_t)cell * D;
+        f2i(sl, x + aso);
+        const double *pc = x + pfo;
+        const int nls = off[cell + 1] - off[cell];
+        double *gc = grad + (size_t)cell * PMAX * dim;
+        for (int g = 0; g < sl[0]; g++)
+            for (int d = 0; d < dim; d++) {
+                gc[g * dim + d] = 0.0;
+                for (int j = 0; j < nls; j++) gc[g * dim + d] -= gw[(size_t)(off[cell] + j) * dim + d] * pc[g];
+            }
+        for (int j = 0; j < nls; j++) {
+            const double *xn = X + (size_t)nbr[off[cell] + j] * D;
+            f2i(nl, xn + aso);
+            orc_set_intersection(is, iL, iR, sl, nl);
+            for (int g = 0; g < is[0]; g++)
+                for (int d = 0; d < dim; d++)
+                    gc[iL[g] * dim + d] += gw[(size_t)(off[cell] + j) * dim + d] * xn[pfo + iR[g]];
+        }
+    }
+}
+
+/* Gradients of every cell's active phase fractions (test access):
+ * grad[(cell*16 + g)*dim + d]. */
+int orc_gradients(const cipher_config *cfg, const cipher_mesh *mesh, const double *X, double *grad)
+{
+    olayout L = orc_layout(cfg);
+    int *off, *nbr;
+    double *gw;
+    if (build_adjacency(mesh, &off, &nbr)) return CIPHER_ERR_ARG;
+    int rc = lsq_setup(cfg, mesh, off, nbr, &gw);
+    if (rc == CIPHER_OK) {
+        lsq_gradients(cfg, mesh, off, nbr, gw, X, L.ndof, L.aso, L.pfo, grad);
+        free(gw);
+    }
+    free(off); free(nbr);
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
 /* RHSFunction, src/cipher.c:27-681                                    */
 /* ------------------------------------------------------------------ */
-#define PMAX 16
 
 int orc_rhs(const cipher_config *cfg, const cipher_mesh *mesh, double t,
             const double *X, double *F, uint32_t *activeout)
@@ -728,8 +849,22 @@ int orc_rhs(const cipher_config *cfg, const cipher_mesh *mesh, double t,
     const int N = mesh->ncells, D = L.ndof, P = L.pf;
     const int nc = cfg->ncp, nd = nc - 1, npf = cfg->npf, dim = cfg->dim;
     const int SF = L.sf, SP = L.sp, DPS = L.dp;
-    if (cfg->gradient_calculation) return CIPHER_ERR_UNSUPPORTED;
     if (P > PMAX) return CIPHER_ERR_ARG;
+    double *grad = NULL;
+    if (cfg->gradient_calculation) {
+        int *goff, *gnbr;
+        double *gw;
+        if (build_adjacency(mesh, &goff, &gnbr)) return CIPHER_ERR_ARG;
+        int rc = lsq_setup(cfg, mesh, goff, gnbr, &gw);
+        if (rc == CIPHER_OK) {
+            grad = calloc((size_t)N * PMAX * dim, sizeof(double));
+            if (grad) lsq_gradients(cfg, mesh, goff, gnbr, gw, X, D, L.aso, L.pfo, grad);
+            else rc = CIPHER_ERR_ARG;
+            free(gw);
+        }
+        free(goff); free(gnbr);
+        if (rc != CIPHER_OK) return rc;
+    }
 
     double *lap = calloc((size_t)N * D, sizeof(double));
     double *sfg = calloc((size_t)N * P * (SF > 0 ? SF : 1), sizeof(double));
@@ -737,7 +872,7 @@ int orc_rhs(const cipher_config *cfg, const cipher_mesh *mesh, double t,
     double *mobg = calloc((size_t)N * nc, sizeof(double));
     double *tcond = calloc((size_t)N, sizeof(double));
     if (!lap || !sfg || !compg || !mobg || !tcond) {
-        free(lap); free(sfg); free(compg); free(mobg); free(tcond);
+        free(lap); free(sfg); free(compg); free(mobg); free(tcond); free(grad);
         return CIPHER_ERR_ARG;
     }
     memset(F, 0, (size_t)N * D * sizeof(double));
@@ -894,6 +1029,28 @@ int orc_rhs(const cipher_config *cfg, const cipher_mesh *mesh, double t,
                     int kj = gk * n + gj;
                     ie[kj] = 1.0;
                     im[kj] = 1.0;
+                    if (grad) {
+                        /* anisotropy, src/cipher.c:429-457 */
+                        const double *gk_ = grad + ((size_t)cell * PMAX + gk) * dim;
+                        const double *gj_ = grad + ((size_t)cell * PMAX + gj) * dim;
+                        double nv[3], rn = 0.0, dot;
+                        for (int d = 0; d < dim; d++) {
+                            nv[d] = gk_[d] - gj_[d];
+                            rn += nv[d] * nv[d];
+                        }
+                        rn = sqrt(rn);
+                        if (rn > 1e-16) for (int d = 0; d < dim; d++) nv[d] /= rn;
+                        for (int r = 0; r < it->e_naniso; r++) {
+                            dot = 0.0;
+                            for (int d = 0; d < dim; d++) dot += nv[d] * cfg->aniso[it->e_aniso + dim * r + d];
+                            ie[kj] += orc_fastpow(dot, 4) * cfg->aniso[it->e_aniso + dim * it->e_naniso + r];
+                        }
+                        for (int r = 0; r < it->m_naniso; r++) {
+                            dot = 0.0;
+                            for (int d = 0; d < dim; d++) dot += nv[d] * cfg->aniso[it->m_aniso + dim * r + d];
+                            im[kj] += orc_fastpow(dot, 4) * cfg->aniso[it->m_aniso + dim * it->m_naniso + r];
+                        }
+                    }
                     ie[kj] *= it->e0;
                     if (cfg->tseries[it->e_act].n) ie[kj] *= exp(orc_sum_tseries(T, TS(it->e_act)) / RG / T);
                     im[kj] *= it->m0;
@@ -1065,36 +1222,8 @@ int orc_rhs(const cipher_config *cfg, const cipher_mesh *mesh, double t,
             matvec(dprhs, dmdc, wdp, DPS);
         }
     }
-    free(lap); free(sfg); free(compg); free(mobg); free(tcond);
+    free(lap); free(sfg); free(compg); free(mobg); free(tcond); free(grad);
     return CIPHER_OK;
-}
-
-/* ------------------------------------------------------------------ */
-/* face adjacency (both directions) used by the set unions              */
-/* ------------------------------------------------------------------ */
-static int build_adjacency(const cipher_mesh *mesh, int **off_out, int **nbr_out)
-{
-    int N = mesh->ncells;
-    int *off = calloc((size_t)N + 1, sizeof(int));
-    int *nbr = malloc((size_t)(2 * mesh->nfaces + 1) * sizeof(int));
-    if (!off || !nbr) { free(off); free(nbr); return -1; }
-    for (int f = 0; f < mesh->nfaces; f++) {
-        off[mesh->face_cells[2 * f] + 1]++;
-        off[mesh->face_cells[2 * f + 1] + 1]++;
-    }
-    for (int c = 0; c < N; c++) off[c + 1] += off[c];
-    int *pos = malloc((size_t)N * sizeof(int));
-    if (!pos) { free(off); free(nbr); return -1; }
-    memcpy(pos, off, (size_t)N * sizeof(int));
-    for (int f = 0; f < mesh->nfaces; f++) {
-        int a = mesh->face_cells[2 * f], b = mesh->face_cells[2 * f + 1];
-        nbr[pos[a]++] = b;
-        nbr[pos[b]++] = a;
-    }
-    free(pos);
-    *off_out = off;
-    *nbr_out = nbr;
-    return 0;
 }
 
 /* PostStep active-set block, src/cipher.c:869-974 */

@@ -43,6 +43,10 @@ void orc_mobility_volumeref(const cipher_config *cfg, double *out, const double 
 
 /* RHSFunction, src/cipher.c:27-681.  X, F: ncells*ndof doubles.  active
  * (optional): bit g of active[cell] = active[g]. */
+/* least-squares phase-field gradients, src/cipher.c:82-146 and 1233-1282:
+ * grad[(cell*16 + g)*dim + d] for the cell's active slots g */
+int orc_gradients(const cipher_config *cfg, const cipher_mesh *mesh, const double *X, double *grad);
+
 int orc_rhs(const cipher_config *cfg, const cipher_mesh *mesh, double t,
             const double *X, double *F, uint32_t *active);
 

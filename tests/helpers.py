@@ -313,8 +313,8 @@ def cpu_solve(wl, X, t0, t_end, dt, params, max_steps=0):
 
 
 GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.abspath(__file__)), "golden")
-REF_CASES = ["bc_chem", "bc_thermal", "gb_precipitate", "grain_growth", "nucleation_7050", "nucleation_crti",
-             "segregation", "sl2_t250", "solidification"]
+REF_CASES = ["anisotropy", "bc_chem", "bc_thermal", "gb_precipitate", "grain_growth", "nucleation_7050",
+             "nucleation_crti", "segregation", "sl2_t250", "solidification"]
 
 
 def ref_case(name):
@@ -350,3 +350,28 @@ def crowded(npf=24, cap=16, nmin=6, nmax=16, res=(16, 16), seed=3):
         X[:, L["tm_offset"]] = 1.0
         _cache[key] = (wl, X.ravel())
     return _cache[key]
+
+
+def aniso_pair_yaml(res=(16, 16), e_aniso=(0.3, -0.2), m_aniso=(0.5, 0.25), gradient=True, scale_iso=False,
+                    minlevel=None):
+    """Two phases, one interface type; with gradient, anisotropic energy and
+    mobility along the axes (values e_aniso / m_aniso for directions x, y);
+    with scale_iso, the isotropic interface whose constants are scaled by
+    (1 + value along x) instead."""
+    txt = W.grain_growth_yaml(res=res, ngrains=2, periodic=False, minlevel=minlevel)
+    e0, m0 = 1.0, 1.0
+    if scale_iso:
+        e0, m0 = e0 * (1.0 + e_aniso[0]), m0 * (1.0 + m_aniso[0])
+    dirs = "[1.0, 0.0, 0.0, 1.0]" if len(res) == 2 else "[1.0, 0.0, 0.0, 0.0, 1.0, 0.0]"
+    old = "    energy :\n      e0: 1.0\n    mobility :\n      m0: 1.0\n"
+    assert old in txt
+    new = f"    energy :\n      e0: {e0!r}\n"
+    if gradient and not scale_iso:
+        new += f"      anisotropy_values: [{e_aniso[0]!r}, {e_aniso[1]!r}]\n      anisotropy_directions: {dirs}\n"
+    new += f"    mobility :\n      m0: {m0!r}\n"
+    if gradient and not scale_iso:
+        new += f"      anisotropy_values: [{m_aniso[0]!r}, {m_aniso[1]!r}]\n      anisotropy_directions: {dirs}\n"
+    txt = txt.replace(old, new)
+    if gradient:
+        txt = txt.replace("interpolation: cubic\n", "interpolation: cubic\n  gradient_calculation: 1\n")
+    return txt

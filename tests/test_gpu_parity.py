@@ -53,6 +53,10 @@ WORKLOADS = {
     # and chemical/thermal Neumann and Dirichlet boundaries
     "chem4_thermal": lambda: H.chem(thermal=True),
 }
+# least-squares gradients with anisotropic energy and mobility on a 3D AMR mesh
+WORKLOADS["aniso3d_amr"] = lambda: H._cache.setdefault(
+    ("aniso3d",), __import__("cipher_amd.workload", fromlist=["Workload"]).Workload(
+        H.aniso_pair_yaml(res=(16, 16, 16), minlevel=2), "aniso3d"))
 # the reference's example configurations (tests/golden, reduced grids)
 for _n in H.REF_CASES:
     WORKLOADS[f"ref_{_n}"] = (lambda n=_n: H.ref_case(n))
@@ -201,40 +205,3 @@ def test_rhs_fp32_mixed_within_1e6(name):
     for f, e in errs.items():
         assert e <= RTOL32, f"{name} field {f}: FP32-mixed rel err {e:.3e}"
     assert np.array_equal(_host(a32), _host(a64))
-
-
-def test_gradient_calculation_is_reported_unsupported():
-    """Anisotropy with LSQ gradients (the reference's Anisotropy example) is not
-    on the GPU path: context creation or the RHS says so."""
-    import os
-    from cipher_amd import workload as W
-    with open(os.path.join(H.GOLDEN, "ref_anisotropy.yaml")) as f:
-        wl = W.Workload(f.read(), "anisotropy")
-    with pytest.raises(abi.CipherError) as e:
-        ctx = _ctx(wl)
-        import torch
-        X = torch.zeros(wl.ncells * wl.layout["ndof"], dtype=torch.float64, device=ctx.device)
-        ctx.rhs(X)
-    assert e.value.code == 5
-
-
-@pytest.mark.parametrize("cap,nmin,nmax", [(8, 5, 8), (16, 6, 16)])
-def test_rhs_large_active_sets(cap, nmin, nmax):
-    """Active sets of 5..8 (capacity 8) and 6..16 (capacity 16) phases in
-    every cell: the 8- and 16-lane paths, neighbour lists longer than a lane
-    group, every pair and triple loop at full length."""
-    import torch
-    wl, X = H.crowded(cap=cap, nmin=nmin, nmax=nmax)
-    print(f"[crowded] cap={cap} n={nmin}..{nmax} cells={wl.ncells}", flush=True)
-    for prec, bar in ((abi.FP64, RTOL64), (abi.FP32_MIXED, RTOL32)):
-        ctx = _ctx(wl, prec)
-        Fo, ao = oracle.rhs(wl.cfg, wl.mesh, X, want_active=True)
-        act = torch.zeros(wl.ncells, dtype=torch.int32, device=ctx.device)
-        Fg = _host(ctx.rhs(_dev(ctx, X), active=act))
-        assert np.array_equal(_host(act).astype(np.uint32), ao)
-        errs = H.field_errors(Fg, Fo, wl.layout)
-        print("   ", prec, errs, flush=True)
-        for f, e in errs.items():
-            assert e <= bar, (prec, f, e)
-        if prec == abi.FP64:
-            assert np.array_equal(Fg, Fo)          # grain growth: identical bits

@@ -617,3 +617,64 @@ def test_poststep_capacity_overflow_is_reported():
     # a cell whose neighbours carry three different phases cannot be held
     _, rc = oracle.poststep(wl.cfg, wl.mesh, X.ravel())
     assert rc == abi.ERR_CAPACITY
+
+
+# --------------------------------------------------------------------------
+# LSQ gradients and anisotropy (src/cipher.c:82-146, 429-469, 1233-1282)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("res,minlevel", [((32, 32), 3), ((16, 16, 16), 2)])
+def test_lsq_gradient_exact_for_linear_fields(res, minlevel):
+    """Least squares reproduces a linear field exactly, on AMR meshes with
+    hanging faces too."""
+    wl = W.Workload(H.aniso_pair_yaml(res=res, minlevel=minlevel), "lsq")
+    assert wl.cfg.gradient_calculation == 1 and np.unique(wl.forest.level).size > 1
+    L = wl.layout
+    X = np.zeros((wl.ncells, L["ndof"]))
+    X[:, 0], X[:, 1], X[:, 2] = 2, 0, 1
+    c = wl.forest.centroids()
+    slope = np.array([0.011, -0.007, 0.004])[:wl.cfg.dim]
+    X[:, L["pf_offset"]] = 0.3 + c @ slope
+    X[:, L["pf_offset"] + 1] = 0.7 - c @ slope
+    X[:, L["tm_offset"]] = 1.0
+    g = oracle.gradients(wl.cfg, wl.mesh, X.ravel())
+    assert np.allclose(g[:, 0, :], slope[None, :], rtol=0, atol=1e-14)
+    assert np.allclose(g[:, 1, :], -slope[None, :], rtol=0, atol=1e-14)
+
+
+def test_lsq_gradient_only_counts_common_phases():
+    """A neighbour without the phase contributes only the minus term: the
+    gradient of a phase present in one cell alone is -sum_j G_j phi."""
+    wl = W.Workload(H.aniso_pair_yaml(res=(8, 8)), "lsq1")
+    L = wl.layout
+    X = np.zeros((wl.ncells, L["ndof"]))
+    X[:, 0], X[:, 1], X[:, L["pf_offset"]] = 1, 0, 1.0
+    X[:, L["tm_offset"]] = 1.0
+    k = 27                                         # an interior cell with phase 1 alone
+    X[k, 0], X[k, 1], X[k, L["pf_offset"]] = 1, 1, 0.8
+    g = oracle.gradients(wl.cfg, wl.mesh, X.ravel())
+    c = wl.forest.centroids()
+    nb = [int(b) for a_, b in wl.mesh.faces if a_ == k] + [int(a_) for a_, b in wl.mesh.faces if b == k]
+    A = c[nb] - c[k]
+    Gm = np.linalg.inv(A.T @ A) @ A.T
+    assert np.allclose(g[k, 0], -Gm.sum(axis=1) * 0.8, atol=1e-14)
+
+
+def test_anisotropy_planar_interface_closed_form():
+    """A planar interface normal to x: the unit gradient difference is the
+    x axis, so the anisotropic energy and mobility equal the isotropic ones
+    scaled by (1 + value along x)."""
+    res = (16, 16)
+    wa = W.Workload(H.aniso_pair_yaml(res=res), "aniso")
+    wi = W.Workload(H.aniso_pair_yaml(res=res, scale_iso=True, gradient=False), "iso")
+    L = wa.layout
+    X = np.zeros((wa.ncells, L["ndof"]))
+    X[:, 0], X[:, 1], X[:, 2] = 2, 0, 1
+    x = wa.forest.centroids()[:, 0]
+    phi = 0.5 * (1.0 - np.tanh((x - 8.0) / 3.0))
+    X[:, L["pf_offset"]] = phi
+    X[:, L["pf_offset"] + 1] = 1.0 - phi
+    X[:, L["tm_offset"]] = 1.0
+    Fa = oracle.rhs(wa.cfg, wa.mesh, X.ravel())
+    Fi = oracle.rhs(wi.cfg, wi.mesh, X.ravel())
+    assert np.abs(Fi).max() > 0
+    assert np.allclose(Fa, Fi, rtol=1e-13, atol=1e-13 * np.abs(Fi).max())

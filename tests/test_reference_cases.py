@@ -68,11 +68,13 @@ def test_reference_case_conserves_composition(name):
     assert (dcs <= 1e-5 * dfixs + 1e-8 * dfixs.max()).all(), (dcs / np.maximum(dfixs, 1e-300)).max()
 
 
-def test_anisotropy_case_is_reported_unsupported():
-    import os
-    from cipher_amd import workload as W
-    with open(os.path.join(H.GOLDEN, "ref_anisotropy.yaml")) as f:
-        wl = W.Workload(f.read(), "anisotropy")
+def test_anisotropy_case_runs():
+    """The reference's anisotropy example (LSQ gradients, anisotropic
+    mobility) through the oracle."""
+    wl = H.ref_case("anisotropy")
     assert wl.cfg.gradient_calculation == 1
-    with pytest.raises(RuntimeError, match="5"):
-        oracle.rhs(wl.cfg, wl.mesh, H.initial_state(wl))
+    X = H.perturb(wl, H.evolve(wl, H.initial_state(wl), steps=3), seed=2)
+    F = oracle.rhs(wl.cfg, wl.mesh, X)
+    assert np.isfinite(F).all()
+    r = F.reshape(-1, wl.layout["ndof"])[:, wl.layout["pf_offset"]:wl.layout["pf_offset"] + wl.layout["pf_size"]]
+    assert (np.abs(r.sum(axis=1)) <= 1e-12 * (np.abs(r).max(axis=1) + 1e-300)).all()
