@@ -39,6 +39,8 @@ def _atoi(v):
     if isinstance(v, (int,)) and not isinstance(v, bool):
         return int(v)
     s = str(v).strip()
+    if s.isdigit():
+        return int(s)
     digits = ""
     for ch in s:
         if ch in "+-" and not digits:
@@ -244,7 +246,10 @@ class CipherConfig:
 
 def load_config(text, loadcase=0):
     """Parse reference-schema YAML text into a CipherConfig."""
-    root = yaml.safe_load(text)
+    # libyaml's safe loader when present (same documents, much faster on the
+    # long RLE mapping blocks); it constructs no arbitrary objects either
+    loader = getattr(yaml, "CSafeLoader", yaml.SafeLoader)
+    root = yaml.load(text, Loader=loader)
     doc = _Doc(root)
     cfg = CipherConfig()
     pools = _Pools()

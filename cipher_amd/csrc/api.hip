@@ -491,7 +491,7 @@ extern "C" int cipher_setup_problem(cipher_ctx *c, const int32_t *phase, double 
 extern "C" int cipher_wrms_norm(cipher_ctx *c, const double *U, const double *Y, int64_t n, double atol, double rtol,
                                 double *norm_out) {
     if (!c || !U || !Y || !norm_out || n < 0) return fail(CIPHER_ERR_ARG, "bad argument");
-    cipher_status s = launch_wrms(U, Y, n, atol, rtol, c->partial, c->npart, c->norm, c->stream);
+    cipher_status s = launch_wrms(U, Y, n, atol, rtol, c->partial, wrms_parts(n), c->norm, c->stream);
     if (s != CIPHER_OK) return fail(s, "wrms launch");
     HIPCHK(hipMemcpyAsync(c->h_norm, c->norm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -9,7 +9,9 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
-SOURCES = ["rhs_nc1.hip", "rhs_nc2.hip", "rhs_nc3.hip", "rhs_nc4.hip", "rhs_nc5.hip", "rhs.hip", "post.hip", "ts.hip", "api.hip", "integrator.hip"]
+SOURCES = ["rhs_nc1.hip", "rhs_nc2.hip", "rhs_nc3.hip", "rhs_nc4.hip", "rhs_nc5.hip",
+           "rhs_batch_nc1.hip", "rhs_batch_nc2.hip", "rhs_batch_nc3.hip", "rhs_batch_nc4.hip", "rhs_batch_nc5.hip",
+           "rhs.hip", "post.hip", "ts.hip", "api.hip", "integrator.hip", "batch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

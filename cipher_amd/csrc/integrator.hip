@@ -151,7 +151,7 @@ static int enqueue_attempt(cipher_ctx *c, const double *X) {
     if (launch_axpy_dt(n, s.Xn, X, s.dt_dev, 3, B, k123, st)) return cipher_fail(CIPHER_ERR_HIP, "stage 4");
     if ((rc = cipher_rhs_impl(c, s.Xn, s.K[3], nullptr))) return rc;
     if (launch_axpy_dt(n, s.Xh, X, s.dt_dev, 4, BH, k1234, st)) return cipher_fail(CIPHER_ERR_HIP, "embedded");
-    if (launch_wrms(s.Xn, s.Xh, n, s.p.atol, s.p.rtol, c->partial, c->npart, c->norm, st))
+    if (launch_wrms(s.Xn, s.Xh, n, s.p.atol, s.p.rtol, c->partial, wrms_parts(n), c->norm, st))
         return cipher_fail(CIPHER_ERR_HIP, "norm");
     return CIPHER_OK;
 }

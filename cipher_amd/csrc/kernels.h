@@ -23,6 +23,16 @@ struct RhsArgs {
     long long ncells;
 };
 
+// cells per binning tile: 256 threads of BIN_CPT consecutive cells
+constexpr int BIN_CPT = 8;
+constexpr int BIN_TILE = 256 * BIN_CPT;
+
+// LDS bytes of a k_rhs block of cpb cells of G lanes
+static inline size_t rhs_lds_bytes(const DevCfg &c, int G, int cpb) {
+    // pair interface types; with gradients also the pair energies/mobilities
+    return (size_t)cpb * G * G * (sizeof(int) + (c.has_grad ? 2 * sizeof(double) : 0));
+}
+
 struct RhsLaunch {
     const DevCfg *cfg;
     const double *X;
@@ -54,6 +64,73 @@ struct PostLaunch {
 };
 cipher_status launch_poststep(const PostLaunch &L, hipStream_t st);
 cipher_status launch_setup(const PostLaunch &L, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// Batched ensembles: independent meshes of different sizes, component counts
+// and active-set capacities processed by shared launches.  A launch's grid is
+// cut into segments, one per member; every block finds its segment (a binary
+// search over the segment offsets, uniform per block) and runs the
+// single-mesh kernel body on that member's mesh, configuration and slice of
+// the concatenated state vectors.
+struct BatchMember {
+    RhsArgs a;                // mesh pointers of the member (X, F, lists set per launch)
+    DevCfg c;                 // the member's configuration tables
+    long long xoff;           // first double of the member in the batch vectors
+    long long celloff;        // first cell of the member in batch per-cell arrays
+    int32_t *bins;            // 5 lists of a.ncells entries
+    int32_t *counts;          // 8 counters: bin sizes, then the recomputation list
+    double *cm;               // per-cell prep records
+    int32_t *glist;           // PostStep scratch, ncells*(P+1)
+    int32_t *err;             // PostStep capacity flag of the member
+    double *partial;          // error-norm partial sums of the member
+    int npart;                // number of partial sums
+    int fold;                 // fold the T-series at the member's first cell
+};
+
+// segment of block b: seg[k] <= b < seg[k+1] (uniform per block)
+__device__ __forceinline__ int seg_find(const int32_t *__restrict__ seg, int nseg, int b) {
+    int lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (seg[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+struct BatchSegs {
+    const int32_t *seg = nullptr;     // nseg+1 block offsets
+    const int32_t *segmem = nullptr;  // member index of each segment
+    int nseg = 0;
+    int nblocks = 0;
+    size_t lds = 0;                   // dynamic LDS bytes (max over members)
+    int threads = 256;                // block size
+};
+
+// one class of members: same component count and lane capacity G
+struct BatchLaunch {
+    const BatchMember *mem;
+    const int32_t *live;      // per-member flag (null: all members)
+    int ncp, G, nbins;
+    BatchSegs fold, prep, bin, list[5], redo, post, dofs;
+    const double *X;
+    double *F;
+    uint32_t *active;
+};
+
+// phase 0: prep records and per-size lists (on st[0]); phase 1: the lists
+// (list b on st[b]); phase 2: FP32-mixed recomputation list (on st[0])
+cipher_status launch_rhs_batch(const BatchLaunch &L, int phase, int precision, const hipStream_t *st);
+cipher_status launch_fold_batch(const BatchLaunch &L, hipStream_t st);
+cipher_status launch_post_batch(const BatchLaunch &L, double *X, hipStream_t st);
+cipher_status launch_wrms_batch(const BatchLaunch &L, const double *U, const double *Y, double atol, double rtol,
+                                double *norms, hipStream_t st);
+cipher_status launch_axpy_dt_batch(const BatchLaunch &L, double *Y, const double *X, const double *dt_dev, int nk,
+                                   const double *A, const double *const *K, hipStream_t st);
+cipher_status launch_copy_batch(const BatchLaunch &L, double *Y, const double *X, const int32_t *mask,
+                                hipStream_t st);
+// partial sums of the error norm over n doubles (fixed for a given n)
+int wrms_parts(long long n);
 
 cipher_status launch_wrms(const double *U, const double *Y, long long n, double atol, double rtol,
                           double *partial, int npart, double *out, hipStream_t st);

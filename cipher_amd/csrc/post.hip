@@ -69,8 +69,9 @@ __device__ __forceinline__ int group_union(int &a, int na, int b, int nb) {
 }
 
 template <int G>
-__global__ __launch_bounds__(256) void k_project(double *X, int32_t *glist, long long ncells, DevCfg c) {
-    const long long cell = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+__device__ __forceinline__ void project_cells(double *X, int32_t *glist, long long ncells, const DevCfg &c,
+                                              long long blk) {
+    const long long cell = (blk * blockDim.x + threadIdx.x) / G;
     const int slot = threadIdx.x & (G - 1);
     if (cell >= ncells) return;
     double *x = X + cell * c.D;
@@ -126,12 +127,18 @@ __global__ __launch_bounds__(256) void k_project(double *X, int32_t *glist, long
     }
 }
 
+template <int G>
+__global__ __launch_bounds__(256) void k_project(double *X, int32_t *glist, long long ncells, DevCfg c) {
+    project_cells<G>(X, glist, ncells, c, blockIdx.x);
+}
+
 template <int G, int NC>
-__global__ __launch_bounds__(256) void k_union(double *X, const int32_t *glist, const int32_t *cell_off,
-                                               const int32_t *adj, int32_t *err, long long ncells, DevCfg c) {
+__device__ __forceinline__ void union_cells(double *X, const int32_t *glist, const int32_t *cell_off,
+                                            const int32_t *adj, int32_t *err, long long ncells, const DevCfg &c,
+                                            long long blk) {
     constexpr int ND = NC - 1;
     constexpr int SPV = 2 * Cnt<ND>::v;
-    const long long cell = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const long long cell = (blk * blockDim.x + threadIdx.x) / G;
     const int slot = threadIdx.x & (G - 1);
     if (cell >= ncells) return;
     double *x = X + cell * c.D;
@@ -192,6 +199,40 @@ __global__ __launch_bounds__(256) void k_union(double *X, const int32_t *glist, 
         x[c.aso + 1 + slot] = (double)ss;
     }
     if (slot == 0) x[c.aso] = (double)cnt;
+}
+
+template <int G, int NC>
+__global__ __launch_bounds__(256) void k_union(double *X, const int32_t *glist, const int32_t *cell_off,
+                                               const int32_t *adj, int32_t *err, long long ncells, DevCfg c) {
+    union_cells<G, NC>(X, glist, cell_off, adj, err, ncells, c, blockIdx.x);
+}
+
+// batched PostStep: one segment of blocks per member (members of one class)
+template <int G>
+__global__ __launch_bounds__(256) void k_project_batch(const BatchMember *__restrict__ mem,
+                                                       const int32_t *__restrict__ live,
+                                                       const int32_t *__restrict__ seg,
+                                                       const int32_t *__restrict__ segmem, int nseg, double *X) {
+    const int k = seg_find(seg, nseg, blockIdx.x);
+    const int mi = segmem[k];
+    if (live && !live[mi]) return;
+    const BatchMember &m = mem[mi];
+    const DevCfg c = m.c;
+    project_cells<G>(X + m.xoff, m.glist, m.a.ncells, c, (long long)blockIdx.x - seg[k]);
+}
+
+template <int G, int NC>
+__global__ __launch_bounds__(256) void k_union_batch(const BatchMember *__restrict__ mem,
+                                                     const int32_t *__restrict__ live,
+                                                     const int32_t *__restrict__ seg,
+                                                     const int32_t *__restrict__ segmem, int nseg, double *X) {
+    const int k = seg_find(seg, nseg, blockIdx.x);
+    const int mi = segmem[k];
+    if (live && !live[mi]) return;
+    const BatchMember &m = mem[mi];
+    const DevCfg c = m.c;
+    union_cells<G, NC>(X + m.xoff, m.glist, m.a.cell_off, m.a.adj, m.err, m.a.ncells, c,
+                       (long long)blockIdx.x - seg[k]);
 }
 
 template <int G, int NC>
@@ -289,6 +330,36 @@ static cipher_status post_dispatch(const PostLaunch &L, hipStream_t st, bool set
     if (P <= 4) return post_G<4>(L, st, setup);
     if (P <= 8) return post_G<8>(L, st, setup);
     if (P <= 16) return post_G<16>(L, st, setup);
+    return CIPHER_ERR_UNSUPPORTED;
+}
+
+template <int G>
+static cipher_status post_batch_G(const BatchLaunch &L, double *X, hipStream_t st) {
+    const BatchSegs &s = L.post;
+    if (s.nblocks == 0) return CIPHER_OK;
+    hipLaunchKernelGGL((k_project_batch<G>), dim3((unsigned)s.nblocks), dim3(256), 0, st, L.mem, L.live, s.seg,
+                       s.segmem, s.nseg, X);
+#define CIPHER_POSTB_NC(NC)                                                                                 \
+    case NC:                                                                                                \
+        hipLaunchKernelGGL((k_union_batch<G, NC>), dim3((unsigned)s.nblocks), dim3(256), 0, st, L.mem, L.live, \
+                           s.seg, s.segmem, s.nseg, X);                                                     \
+        break;
+    switch (L.ncp) {
+        CIPHER_POSTB_NC(1)
+        CIPHER_POSTB_NC(2)
+        CIPHER_POSTB_NC(3)
+        CIPHER_POSTB_NC(4)
+        CIPHER_POSTB_NC(5)
+    default: return CIPHER_ERR_UNSUPPORTED;
+    }
+#undef CIPHER_POSTB_NC
+    return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
+}
+
+cipher_status launch_post_batch(const BatchLaunch &L, double *X, hipStream_t st) {
+    if (L.G == 4) return post_batch_G<4>(L, X, st);
+    if (L.G == 8) return post_batch_G<8>(L, X, st);
+    if (L.G == 16) return post_batch_G<16>(L, X, st);
     return CIPHER_ERR_UNSUPPORTED;
 }
 

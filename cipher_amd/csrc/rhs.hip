@@ -19,6 +19,20 @@ cipher_status launch_rhs(const RhsLaunch &L, int precision, hipStream_t st) {
     }
 }
 
+template <int NC>
+cipher_status launch_batch_NC(const BatchLaunch &L, int phase, int precision, const hipStream_t *st);
+
+cipher_status launch_rhs_batch(const BatchLaunch &L, int phase, int precision, const hipStream_t *st) {
+    switch (L.ncp) {
+    case 1: return launch_batch_NC<1>(L, phase, precision, st);
+    case 2: return launch_batch_NC<2>(L, phase, precision, st);
+    case 3: return launch_batch_NC<3>(L, phase, precision, st);
+    case 4: return launch_batch_NC<4>(L, phase, precision, st);
+    case 5: return launch_batch_NC<5>(L, phase, precision, st);
+    default: return CIPHER_ERR_UNSUPPORTED;
+    }
+}
+
 // fold every T-series at the temperature of the first cell (deterministic:
 // the same code as the direct evaluation)
 __global__ void k_tsfold(const double *X, long long ncells, DevCfg c, double *tsv) {
@@ -29,6 +43,28 @@ __global__ void k_tsfold(const double *X, long long ncells, DevCfg c, double *ts
 
 cipher_status launch_tsfold(const double *X, long long ncells, const DevCfg &c, double *tsv, hipStream_t st) {
     hipLaunchKernelGGL(k_tsfold, dim3(1), dim3(256), 0, st, X, ncells, c, tsv);
+    return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
+}
+
+// the same per member (one block each) at the member's first cell
+__global__ void k_tsfold_batch(const BatchMember *__restrict__ mem, const int32_t *__restrict__ live,
+                               const int32_t *__restrict__ seg, const int32_t *__restrict__ segmem, int nseg,
+                               const double *X) {
+    const int mi = segmem[seg_find(seg, nseg, blockIdx.x)];
+    if (live && !live[mi]) return;
+    const BatchMember &m = mem[mi];
+    if (!m.fold) return;
+    const DevCfg &c = m.c;
+    const double T = m.a.ncells > 0 ? X[m.xoff + c.tmo] : 0.0;
+    double *tsv = const_cast<double *>(c.tsv);
+    for (int i = threadIdx.x; i < c.ntseries; i += blockDim.x) tsv[i] = dtseries<double>(&c.ts[i], T);
+    if (threadIdx.x == 0) tsv[c.ntseries] = T;
+}
+
+cipher_status launch_fold_batch(const BatchLaunch &L, hipStream_t st) {
+    if (L.fold.nblocks == 0) return CIPHER_OK;
+    hipLaunchKernelGGL(k_tsfold_batch, dim3((unsigned)L.fold.nblocks), dim3(256), 0, st, L.mem, L.live, L.fold.seg,
+                       L.fold.segmem, L.fold.nseg, L.X);
     return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
 }
 

@@ -155,13 +155,14 @@ __device__ __forceinline__ void stage_in(double *lds, const double *X, long long
 
 // ---------------------------------------------------------------------------
 // k_prep: composition, mobility, conductivity per cell (src/cipher.c:149-233)
+// (one block of CPB cells; blk is the block's index within the mesh)
 template <int G, int NC>
-__global__ __launch_bounds__(256) void k_prep(const double *__restrict__ X, double *__restrict__ cm,
-                                              long long ncells, DevCfg c) {
+__device__ __forceinline__ void prep_cells(const double *__restrict__ X, double *__restrict__ cm, long long ncells,
+                                           const DevCfg &c, long long blk) {
     constexpr int ND = NC - 1;
     constexpr int CPB = 256 / G;
     extern __shared__ double lds[];
-    const long long c0 = (long long)blockIdx.x * CPB;
+    const long long c0 = blk * CPB;
     const int nblk = (int)min((long long)CPB, ncells - c0);
     stage_in(lds, X, c0, nblk, c.D);
     __syncthreads();
@@ -245,6 +246,12 @@ __global__ __launch_bounds__(256) void k_prep(const double *__restrict__ X, doub
     }
 }
 
+template <int G, int NC>
+__global__ __launch_bounds__(256) void k_prep(const double *__restrict__ X, double *__restrict__ cm,
+                                              long long ncells, DevCfg c) {
+    prep_cells<G, NC>(X, cm, ncells, c, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // interface energy / mobility of a pair from the compact [e0, width, m0]
 // table; an activation factor exp(Q(T)/RT) is evaluated in FP64 (Q/RT is
@@ -278,8 +285,9 @@ __device__ __forceinline__ void wave_sync() {
 // stages its cell's record in LDS, computes, and writes the cell's F record
 // itself.  In FP32-mixed, cells whose active flags fall inside the FP32
 // rounding band are appended to a.redo for the FP64 pass.
+// (blocks blk, blk + nblk, ... of the list; a block is 256/G groups)
 template <int G, int GMAX, int NC, class R>
-__global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg c) {
+__device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, long long blk, long long nblk) {
     constexpr int RM = (GMAX + G - 1) / G;                 // neighbour entries per lane
     constexpr int ND = NC - 1;
     constexpr int NDV = Cnt<ND>::v;
@@ -292,10 +300,10 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
     const int slot = threadIdx.x & (G - 1);
     int *pt = ptype + lc * G * G;
     const long long limit = (long long)*a.cells_count;
-    const long long stride = (long long)gridDim.x * CPB;
+    const long long stride = nblk * CPB;
     // the G lanes of a group share the loop trip count; groups of a wave may
     // differ, so nothing below synchronises more than the group's wave
-    for (long long j = (long long)blockIdx.x * CPB + lc; j < limit; j += stride) {
+    for (long long j = blk * CPB + lc; j < limit; j += stride) {
         const long long cell = (long long)a.cells[j];
         if constexpr (G == 1 && NC == 1) {
             // single-phase cell without chemistry, conduction or heat terms:
@@ -1018,6 +1026,11 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg 
     }  // cell loop
 }
 
+template <int G, int GMAX, int NC, class R>
+__global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg c) {
+    rhs_cells<G, GMAX, NC, R>(a, c, blockIdx.x, gridDim.x);
+}
+
 // bin b holds the cells whose active-set size needs 2^b lanes
 __device__ __forceinline__ int lane_bin(int n) {
     return n <= 1 ? 0 : (n == 2 ? 1 : (n <= 4 ? 2 : (n <= 8 ? 3 : 4)));
@@ -1028,15 +1041,12 @@ __device__ __forceinline__ int lane_bin(int n) {
 // of the per-thread counts gives every cell its place, and one atomic per bin
 // per tile reserves the tile's range, so each list keeps cell order within a
 // tile (and the Morton locality of the tiles).
-constexpr int BIN_CPT = 8;
-constexpr int BIN_TILE = 256 * BIN_CPT;
-template <int U>
-__global__ __launch_bounds__(256) void k_bin(const double *__restrict__ X, long long ncells, DevCfg c, int nbins,
-                                              int32_t *__restrict__ bins, long long binstride,
-                                              int32_t *__restrict__ counts) {
+__device__ __forceinline__ void bin_tile(const double *__restrict__ X, long long ncells, const DevCfg &c, int nbins,
+                                         int32_t *__restrict__ bins, long long binstride,
+                                         int32_t *__restrict__ counts, long long blk) {
     __shared__ int scan[5][256];
     __shared__ int base[5];
-    const long long c0 = (long long)blockIdx.x * BIN_TILE + (long long)threadIdx.x * BIN_CPT;
+    const long long c0 = blk * BIN_TILE + (long long)threadIdx.x * BIN_CPT;
     int bq[BIN_CPT];
     int cnt[5] = {0, 0, 0, 0, 0};
 #pragma unroll
@@ -1079,15 +1089,17 @@ __global__ __launch_bounds__(256) void k_bin(const double *__restrict__ X, long 
 }
 
 template <int U>
+__global__ __launch_bounds__(256) void k_bin(const double *__restrict__ X, long long ncells, DevCfg c, int nbins,
+                                              int32_t *__restrict__ bins, long long binstride,
+                                              int32_t *__restrict__ counts) {
+    bin_tile(X, ncells, c, nbins, bins, binstride, counts, blockIdx.x);
+}
+
+template <int U>
 __global__ void k_fill_i32(int32_t *p, int n, int32_t v) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = v;
 }
 
-// LDS bytes of a k_rhs block of cpb cells
-static inline size_t rhs_lds_bytes(const DevCfg &c, int G, int cpb) {
-    // pair interface types; with gradients also the pair energies/mobilities
-    return (size_t)cpb * G * G * (sizeof(int) + (c.has_grad ? 2 * sizeof(double) : 0));
-}
 
 // One k_rhs launch over a device-counted cell list: blocks of 256 threads in
 // a grid sized for the whole mesh (the list length is only known on the

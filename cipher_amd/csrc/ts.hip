@@ -17,12 +17,13 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_wrms_partial(const double *__restrict__ U, const double *__restrict__ Y,
-                                                      long long n, double atol, double rtol, double *partial) {
+// block blk of nblk: grid-stride sum over n doubles into partial[blk]
+__device__ __forceinline__ void wrms_part(const double *__restrict__ U, const double *__restrict__ Y, long long n,
+                                          double atol, double rtol, double *partial, long long blk, long long nblk) {
     __shared__ double red[4];
     double s = 0.0;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const long long stride = nblk * blockDim.x;
+    for (long long i = blk * blockDim.x + threadIdx.x; i < n; i += stride) {
         const double u = U[i], y = Y[i];
         const double au = fabs(u), ay = fabs(y);
         const double tol = atol + rtol * (au > ay ? au : ay);
@@ -33,10 +34,15 @@ __global__ __launch_bounds__(256) void k_wrms_partial(const double *__restrict__
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) red[w] = s;
     __syncthreads();
-    if (threadIdx.x == 0) partial[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+    if (threadIdx.x == 0) partial[blk] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-__global__ __launch_bounds__(256) void k_wrms_final(const double *partial, int npart, long long n, double *out) {
+__global__ __launch_bounds__(256) void k_wrms_partial(const double *__restrict__ U, const double *__restrict__ Y,
+                                                      long long n, double atol, double rtol, double *partial) {
+    wrms_part(U, Y, n, atol, rtol, partial, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void wrms_fin(const double *partial, int npart, long long n, double *out) {
     __shared__ double red[4];
     double s = 0.0;
     for (int i = threadIdx.x; i < npart; i += blockDim.x) s += partial[i];
@@ -48,6 +54,51 @@ __global__ __launch_bounds__(256) void k_wrms_final(const double *partial, int n
         const double tot = ((red[0] + red[1]) + red[2]) + red[3];
         out[0] = n > 0 ? sqrt(tot / (double)n) : 0.0;
     }
+}
+
+__global__ __launch_bounds__(256) void k_wrms_final(const double *partial, int npart, long long n, double *out) {
+    wrms_fin(partial, npart, n, out);
+}
+
+int wrms_parts(long long n) {
+    long long p = (n + 4095) / 4096;
+    return (int)(p < 1 ? 1 : (p > 1024 ? 1024 : p));
+}
+
+// per-member norms of a batch: the member's partial sums over its own
+// segment of blocks (the same partition as a single-mesh call), then one
+// final block per member
+__global__ __launch_bounds__(256) void k_wrms_partial_batch(const BatchMember *__restrict__ mem,
+                                                            const int32_t *__restrict__ live,
+                                                            const int32_t *__restrict__ seg,
+                                                            const int32_t *__restrict__ segmem, int nseg,
+                                                            const double *U, const double *Y, double atol, double rtol) {
+    const int k = seg_find(seg, nseg, blockIdx.x);
+    const int mi = segmem[k];
+    if (live && !live[mi]) return;
+    const BatchMember &m = mem[mi];
+    const long long n = m.a.ncells * (long long)m.c.D;
+    wrms_part(U + m.xoff, Y + m.xoff, n, atol, rtol, m.partial, (long long)blockIdx.x - seg[k],
+              (long long)(seg[k + 1] - seg[k]));
+}
+
+__global__ __launch_bounds__(256) void k_wrms_final_batch(const BatchMember *__restrict__ mem,
+                                                          const int32_t *__restrict__ live,
+                                                          const int32_t *__restrict__ segmem, double *norms) {
+    const int mi = segmem[blockIdx.x];
+    if (live && !live[mi]) return;
+    const BatchMember &m = mem[mi];
+    wrms_fin(m.partial, m.npart, m.a.ncells * (long long)m.c.D, norms + mi);
+}
+
+cipher_status launch_wrms_batch(const BatchLaunch &L, const double *U, const double *Y, double atol, double rtol,
+                                double *norms, hipStream_t st) {
+    const BatchSegs &s = L.dofs;
+    if (s.nblocks == 0) return CIPHER_OK;
+    hipLaunchKernelGGL(k_wrms_partial_batch, dim3((unsigned)s.nblocks), dim3(256), 0, st, L.mem, L.live, s.seg,
+                       s.segmem, s.nseg, U, Y, atol, rtol);
+    hipLaunchKernelGGL(k_wrms_final_batch, dim3((unsigned)s.nseg), dim3(256), 0, st, L.mem, L.live, s.segmem, norms);
+    return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
 }
 
 cipher_status launch_wrms(const double *U, const double *Y, long long n, double atol, double rtol,
@@ -120,6 +171,80 @@ __global__ __launch_bounds__(256) void k_axpy_dt(long long n, double *__restrict
         for (int j = 1; j < NK; j++) t += a[j] * p.K[j][i];
         Y[i] = X[i] + t;
     }
+}
+
+// per-member step sizes: member mi's slice uses dtp[mi]
+template <int NK>
+__global__ __launch_bounds__(256) void k_axpy_dt_batch(const BatchMember *__restrict__ mem,
+                                                       const int32_t *__restrict__ live,
+                                                       const int32_t *__restrict__ seg,
+                                                       const int32_t *__restrict__ segmem, int nseg, double *Y,
+                                                       const double *X, const double *__restrict__ dtp, AxpyArgs p) {
+    const int k = seg_find(seg, nseg, blockIdx.x);
+    const int mi = segmem[k];
+    if (live && !live[mi]) return;
+    const BatchMember &m = mem[mi];
+    const long long n = m.a.ncells * (long long)m.c.D;
+    const long long o = m.xoff;
+    const double dt = dtp[mi];
+    double a[NK];
+#pragma unroll
+    for (int j = 0; j < NK; j++) a[j] = dt * p.a[j];
+    const long long stride = (long long)(seg[k + 1] - seg[k]) * blockDim.x;
+    for (long long i = ((long long)blockIdx.x - seg[k]) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double t = a[0] * p.K[0][o + i];
+#pragma unroll
+        for (int j = 1; j < NK; j++) t += a[j] * p.K[j][o + i];
+        Y[o + i] = X[o + i] + t;
+    }
+}
+
+cipher_status launch_axpy_dt_batch(const BatchLaunch &L, double *Y, const double *X, const double *dt_dev, int nk,
+                                   const double *A, const double *const *K, hipStream_t st) {
+    if (nk < 1 || nk > 4) return CIPHER_ERR_ARG;
+    const BatchSegs &s = L.dofs;
+    if (s.nblocks == 0) return CIPHER_OK;
+    AxpyArgs p;
+    for (int j = 0; j < 8; j++) {
+        p.K[j] = j < nk ? K[j] : nullptr;
+        p.a[j] = j < nk ? A[j] : 0.0;
+    }
+#define CIPHER_AXB(NK)                                                                                       \
+    hipLaunchKernelGGL((k_axpy_dt_batch<NK>), dim3((unsigned)s.nblocks), dim3(256), 0, st, L.mem, L.live, s.seg, \
+                       s.segmem, s.nseg, Y, X, dt_dev, p)
+    switch (nk) {
+    case 1: CIPHER_AXB(1); break;
+    case 2: CIPHER_AXB(2); break;
+    case 3: CIPHER_AXB(3); break;
+    default: CIPHER_AXB(4); break;
+    }
+#undef CIPHER_AXB
+    return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
+}
+
+// Y = X on the slices of the members whose mask entry is set
+__global__ __launch_bounds__(256) void k_copy_batch(const BatchMember *__restrict__ mem,
+                                                    const int32_t *__restrict__ seg,
+                                                    const int32_t *__restrict__ segmem, int nseg, double *Y,
+                                                    const double *X, const int32_t *__restrict__ mask) {
+    const int k = seg_find(seg, nseg, blockIdx.x);
+    const int mi = segmem[k];
+    if (!mask[mi]) return;
+    const BatchMember &m = mem[mi];
+    const long long n = m.a.ncells * (long long)m.c.D;
+    const long long o = m.xoff;
+    const long long stride = (long long)(seg[k + 1] - seg[k]) * blockDim.x;
+    for (long long i = ((long long)blockIdx.x - seg[k]) * blockDim.x + threadIdx.x; i < n; i += stride)
+        Y[o + i] = X[o + i];
+}
+
+cipher_status launch_copy_batch(const BatchLaunch &L, double *Y, const double *X, const int32_t *mask,
+                                hipStream_t st) {
+    const BatchSegs &s = L.dofs;
+    if (s.nblocks == 0) return CIPHER_OK;
+    hipLaunchKernelGGL(k_copy_batch, dim3((unsigned)s.nblocks), dim3(256), 0, st, L.mem, s.seg, s.segmem, s.nseg, Y,
+                       X, mask);
+    return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
 }
 
 cipher_status launch_axpy_dt(long long n, double *Y, const double *X, const double *dt_dev, int nk, const double *A,

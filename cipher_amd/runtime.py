@@ -144,6 +144,141 @@ class Context:
             pass
 
 
+class Batch:
+    """Many contexts run through shared launches (``cipher_batch``).  Batch
+    state tensors hold the members' records back to back; ``offsets[i]`` is
+    member i's first double, ``cell_offsets[i]`` its first cell.  The members
+    must stay alive and unused on their own while the batch exists."""
+
+    def __init__(self, members, precision=abi.FP64):
+        import torch
+        self.torch = torch
+        self.lib = abi.load_library()
+        self.members = list(members)
+        n = len(self.members)
+        hs = (C.c_void_p * n)(*[m.h.value for m in self.members])
+        h = C.c_void_p()
+        abi.check(self.lib.cipher_batch_create(hs, n, C.byref(h)), "cipher_batch_create")
+        self.h = h
+        self.device = self.members[0].device
+        off = (C.c_int64 * (n + 1))()
+        coff = (C.c_int64 * (n + 1))()
+        abi.check(self.lib.cipher_batch_offsets(h, off, coff), "cipher_batch_offsets")
+        self.offsets = list(off)
+        self.cell_offsets = list(coff)
+        self.set_precision(precision)
+
+    def __len__(self):
+        return len(self.members)
+
+    def _stream(self):
+        s = self.torch.cuda.current_stream(self.device).cuda_stream
+        abi.check(self.lib.cipher_batch_set_stream(self.h, C.c_void_p(s)), "cipher_batch_set_stream")
+
+    def set_precision(self, precision):
+        abi.check(self.lib.cipher_batch_set_precision(self.h, precision), "cipher_batch_set_precision")
+        self.precision = precision
+
+    def _check(self, X, what):
+        if X.dtype != self.torch.float64 or not X.is_cuda or not X.is_contiguous():
+            raise ValueError(f"{what}: expected a contiguous float64 device tensor")
+        if X.numel() != self.offsets[-1]:
+            raise ValueError(f"{what}: expected {self.offsets[-1]} doubles, got {X.numel()}")
+
+    def empty_state(self):
+        return self.torch.zeros(self.offsets[-1], dtype=self.torch.float64, device=self.device)
+
+    def slice(self, X, i):
+        return X[self.offsets[i]:self.offsets[i + 1]]
+
+    def rhs(self, X, F=None, t=0.0, active=None):
+        self._check(X, "X")
+        if F is None:
+            F = self.torch.empty_like(X)
+        self._check(F, "F")
+        self._stream()
+        if active is None:
+            rc = self.lib.cipher_batch_rhs(self.h, t, _ptr(X), _ptr(F))
+        else:
+            if active.dtype != self.torch.int32 or active.numel() != self.cell_offsets[-1]:
+                raise ValueError("active: expected int32 tensor of the batch's cells")
+            rc = self.lib.cipher_batch_rhs_active(self.h, t, _ptr(X), _ptr(F), _ptr(active))
+        abi.check(rc, "cipher_batch_rhs")
+        return F
+
+    def poststep(self, X):
+        self._check(X, "X")
+        self._stream()
+        bad = C.c_int32(-1)
+        rc = self.lib.cipher_batch_poststep(self.h, _ptr(X), C.byref(bad))
+        if rc:
+            err = abi.CipherError(rc, "cipher_batch_poststep")
+            err.member = bad.value
+            raise err
+        return X
+
+    def wrms_norms(self, U, Y, atol, rtol):
+        n = len(self.members)
+        out = (C.c_double * n)()
+        self._stream()
+        abi.check(self.lib.cipher_batch_wrms_norm(self.h, _ptr(U), _ptr(Y), atol, rtol, out),
+                  "cipher_batch_wrms_norm")
+        return list(out)
+
+    def autotune(self, X, reps=5):
+        """Pick launch shapes by timing on X; returns (ms per batched RHS,
+        [(components, lanes, list, threads, cap), ...])."""
+        self._check(X, "X")
+        F = self.torch.empty_like(X)
+        self._stream()
+        ms = C.c_double()
+        abi.check(self.lib.cipher_batch_autotune(self.h, _ptr(X), _ptr(F), reps, C.byref(ms)),
+                  "cipher_batch_autotune")
+        return ms.value, self.shapes()
+
+    def shapes(self):
+        out = []
+        k = 0
+        while True:
+            v = [C.c_int32() for _ in range(4)]
+            if self.lib.cipher_batch_shape(self.h, k, 0, *[C.byref(x) for x in v]):
+                break
+            ncp, lanes = v[0].value, v[1].value
+            nl = 5 if lanes >= 16 else (4 if lanes >= 8 else 3)
+            for q in range(nl):
+                abi.check(self.lib.cipher_batch_shape(self.h, k, q, *[C.byref(x) for x in v]), "cipher_batch_shape")
+                out.append((ncp, lanes, q, v[2].value, v[3].value))
+            k += 1
+        return out
+
+    def ts_setup(self, params):
+        abi.check(self.lib.cipher_batch_ts_setup(self.h, C.byref(params)), "cipher_batch_ts_setup")
+
+    def ts_solve(self, X, t0, t_end, dt, max_steps=0):
+        """Integrate every member in place; t0, t_end, dt are per-member
+        sequences.  Returns (list of stats, list of next dt)."""
+        self._check(X, "X")
+        self._stream()
+        n = len(self.members)
+        a0 = (C.c_double * n)(*t0)
+        a1 = (C.c_double * n)(*t_end)
+        d = (C.c_double * n)(*dt)
+        st = (abi.TsStats * n)()
+        abi.check(self.lib.cipher_batch_ts_solve(self.h, _ptr(X), a0, a1, d, max_steps, st), "cipher_batch_ts_solve")
+        return list(st), list(d)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cipher_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def RHSFunction(ts, ftime, X, F, user):
     """Mirror of RHSFunction(TS,PetscReal,Vec,Vec,void*) (src/cipher.c:27):
     user is a Context; X and F are device state tensors."""

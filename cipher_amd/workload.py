@@ -198,6 +198,133 @@ def ternary_ppt_yaml(res=(128, 128, 128), ngrains=100, nppt=40, radius=3, capaci
     return text
 
 
+def solution_2d_yaml(res=(128, 128), ngrains=50, ncomp=3, ntypes=3, capacity=8, minlevel=None, seed=SEED,
+                     temperature=700.0, t_end=1.0e-3, dx=1.0e-9):
+    """2D grand-canonical microstructure with ncomp components and ntypes
+    phase types: 'sol0' and 'sol1' are calphaddis solutions with
+    Redlich-Kister binaries (L0 ~ U[-20,20] kJ/mol, L1 ~ U[-5,5] kJ/mol per
+    pair), 'cef' a calphad2sl (.)3(.)1 phase whose end members with the last
+    component on the second sublattice are drawn in [-30,-10] kJ/mol (others
+    carry a 20 kJ/mol penalty).  Mobilities log-uniform in [1e-17,1e-15];
+    grains are Voronoi cells of a periodic domain, assigned to the phase
+    types with weights 0.7/0.15/0.15; mu starts uniform (uniform_mu_override)."""
+    if not 2 <= ncomp <= 4 or not 1 <= ntypes <= 3:
+        raise ValueError("2 <= ncomp <= 4 and 1 <= ntypes <= 3")
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dim = len(res)
+    per = [True] * dim
+    labels = voronoi_labels(res, ngrains, rng, per)
+    comps = ["A", "B", "C", "D"][:ncomp]
+    RT = 8.314462 * temperature
+    lmax = int(round(math.log2(res[0])))
+    minlevel = lmax if minlevel is None else minlevel
+    w = np.array([0.7, 0.15, 0.15][:ntypes])
+    gtype = rng.choice(ntypes, size=ngrains, p=w / w.sum())
+    gtype[0] = 0
+    names = ["sol0", "sol1", "cef"][:ntypes]
+    nominal = {2: [0.9, 0.1], 3: [0.90, 0.07, 0.03], 4: [0.88, 0.07, 0.03, 0.02]}[ncomp]
+
+    def mob_block(ind):
+        M = 10.0 ** rng.uniform(-17.0, -15.0, size=ncomp)
+        out = " " * ind + "mobilityc :\n"
+        for c, nm in enumerate(comps):
+            out += " " * (ind + 2) + f"{nm} :\n" + " " * (ind + 4) + f"mobility0 : {float(M[c] * RT)!r}\n"
+        return out
+
+    def head(name, model):
+        return (f"  {name}:\n    chemicalenergy : {model}\n    molarvolume : 1e-5\n"
+                f"    temperature0: {float(temperature)!r}\n    chempot_ex_kineticcoeff : 1.0\n")
+
+    text = _header(list(res), [r * dx for r in res], ngrains, names,
+                   [f"if{a}{b}" for a in range(ntypes) for b in range(a, ntypes)], comps)
+    text += _solution([t_end], [1] * dim, lmax, minlevel, lmax, capacity, 1e-12, t_end, per)
+    text += "material:\n"
+    for t in range(min(ntypes, 2)):
+        c0 = list(nominal) if t == 0 else [nominal[0] - 0.1] + [v + 0.1 / (ncomp - 1) for v in nominal[1:]]
+        text += head(names[t], "calphaddis") + f"    c0 : {_flow([round(v, 6) for v in c0])}\n" + mob_block(4)
+        text += "    binary_enthalpy :\n"
+        for i in range(ncomp):
+            for j in range(i + 1, ncomp):
+                L0 = rng.uniform(-20e3, 20e3)
+                L1 = rng.uniform(-5e3, 5e3)
+                text += (f"      {comps[i]} :\n        {comps[j]} :\n          nrk : 2\n"
+                         f"          rk_0 :\n            t_coefficient : {float(L0)!r}\n            t_exponent : 0\n"
+                         f"          rk_1 :\n            t_coefficient : {float(L1)!r}\n            t_exponent : 0\n")
+    if ntypes == 3:
+        last = ncomp - 1
+        c0p = [0.01] * ncomp
+        c0p[0] = 1.0 - 0.01 * (ncomp - 1)
+        c0q = [0.01] * ncomp
+        c0q[last] = 1.0 - 0.01 * (ncomp - 1)
+        text += head("cef", "calphad2sl") + "    stochiometry : [0.75, 0.25]\n"
+        text += f"    c0_p : {_flow([round(v, 6) for v in c0p])}\n    c0_q : {_flow([round(v, 6) for v in c0q])}\n"
+        text += mob_block(4) + "    unary_enthalpy :\n"
+        for cp in range(ncomp):
+            for cq in range(ncomp):
+                g = rng.uniform(-30e3, -10e3) if (cq == last and cp < last) else 20e3
+                text += (f"      ({comps[cp]}:{comps[cq]}) :\n        t_coefficient : {float(g)!r}\n"
+                         f"        t_exponent : 0\n")
+    text += "\ninterface:\n"
+    for a in range(ntypes):
+        for b in range(a, ntypes):
+            e0 = 0.5 if a == b else 0.3
+            text += (f"  if{a}{b}:\n    width : {float(4.0 * dx)!r}\n    energy :\n      e0: {e0}\n"
+                     f"    mobility :\n      m0: 1.0e-13\n")
+    text += "\n"
+    pair_id = {}
+    k = 0
+    for a in range(ntypes):
+        for b in range(a, ntypes):
+            pair_id[(a, b)] = k
+            k += 1
+    lut = np.zeros((ntypes, ntypes), dtype=np.int64)
+    for (a, b), v in pair_id.items():
+        lut[a, b] = lut[b, a] = v
+
+    def ptype(i, j):
+        return lut[gtype[i], gtype[j]] + 1
+
+    pm = "\n".join(f"1 of {int(t) + 1}" for t in gtype)
+    text += "mappings:\n  phase_material_mapping : |-\n" + _block(pm, 4) + "\n"
+    text += "  voxel_phase_mapping: |-\n" + _block(_voxel_rle(labels), 4) + "\n"
+    text += "  interface_mapping: |-\n" + _block(_interface_rle(ngrains, ptype), 4) + "\n"
+    return text
+
+
+def ensemble_specs(n=96, seed=SEED, min_side=64, max_side=1024, max_grains=2000):
+    """Instances of the batched ensemble: side log-uniform (a power of two
+    in [min_side, max_side]; AMR from three levels below the finest),
+    grains log-uniform in [10, max_grains] (at most one per 64 voxels),
+    components uniform in 1..4, phase types uniform in 1..3 (1 without
+    chemistry), capacity 8 (every eighth instance 4, every twelfth 16)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lo, hi = int(round(math.log2(min_side))), int(round(math.log2(max_side)))
+    out = []
+    for i in range(n):
+        k = int(rng.integers(lo, hi + 1))
+        side = 1 << k
+        g = int(round(math.exp(rng.uniform(math.log(10), math.log(max_grains)))))
+        g = max(10, min(g, side * side // 64))
+        nc = int(rng.integers(1, 5))
+        nt = int(rng.integers(1, 4)) if nc > 1 else 1
+        cap = 4 if i % 8 == 7 else (16 if i % 12 == 11 else 8)
+        out.append({"side": side, "grains": g, "components": nc, "types": nt, "capacity": cap,
+                    "minlevel": max(k - 3, 3), "seed": seed + 7919 * (i + 1)})
+    return out
+
+
+def ensemble_member(spec):
+    """Workload of one ensemble instance (see ensemble_specs)."""
+    res = (spec["side"], spec["side"])
+    if spec["components"] == 1:
+        txt = grain_growth_yaml(res=res, ngrains=spec["grains"], periodic=True, capacity=spec["capacity"],
+                                minlevel=spec["minlevel"], seed=spec["seed"])
+    else:
+        txt = solution_2d_yaml(res=res, ngrains=spec["grains"], ncomp=spec["components"], ntypes=spec["types"],
+                               capacity=spec["capacity"], minlevel=spec["minlevel"], seed=spec["seed"])
+    return Workload(txt, "ensemble_{side}_{grains}g_{components}c_{types}t".format(**spec))
+
+
 class Workload:
     """Parsed config + leaf forest + flattened mesh + per-cell phase labels."""
 

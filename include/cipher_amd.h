@@ -317,6 +317,61 @@ int cipher_ts_accept(cipher_ctx *ctx, double *X_dev);
 int cipher_ts_solve(cipher_ctx *ctx, double *X_dev, double t0, double t_end, double *dt_inout,
                     int64_t max_steps, cipher_ts_stats *stats);
 
+/* ---- batched ensembles ---------------------------------------------------
+ * No counterpart in the reference, which integrates one microstructure per
+ * process.  A batch runs many independent contexts (each with its own
+ * configuration and uploaded mesh: different sizes, component counts and
+ * capacities) through shared launches: members with the same component count
+ * and lane capacity form a class, and each kernel is one launch per class and
+ * cell list, its grid cut into one segment per member.  Each member's results
+ * are those of its context run alone.  Batch state vectors hold the members'
+ * records back to back; member i starts at offsets[i] doubles.  A batch
+ * borrows its members' mesh and scratch buffers: while it exists, do not run
+ * the members' own RHS/PostStep/integrator or re-upload their meshes. */
+typedef struct cipher_batch cipher_batch;
+
+int cipher_batch_create(cipher_ctx *const *members, int32_t nmembers, cipher_batch **out);
+int cipher_batch_destroy(cipher_batch *b);
+/* offsets[nmembers+1]: first double of each member in a batch vector;
+ * cell_offsets[nmembers+1] (may be NULL): first cell, for per-cell arrays. */
+int cipher_batch_offsets(const cipher_batch *b, int64_t *offsets, int64_t *cell_offsets);
+int cipher_batch_set_precision(cipher_batch *b, int precision);
+int cipher_batch_set_stream(cipher_batch *b, void *stream);
+
+/* Every member's cipher_rhs (cipher_rhs_active) in one batched launch set;
+ * active_dev is indexed by batch cell (cell_offsets). */
+int cipher_batch_rhs(cipher_batch *b, double t, const double *X_dev, double *F_dev);
+int cipher_batch_rhs_active(cipher_batch *b, double t, const double *X_dev, double *F_dev,
+                            uint32_t *active_dev);
+
+/* Every member's cipher_poststep; on CIPHER_ERR_CAPACITY *failed_member (may
+ * be NULL) names the first member whose union overflowed, else -1. */
+int cipher_batch_poststep(cipher_batch *b, double *X_dev, int32_t *failed_member);
+
+/* norms[i] (host) = cipher_wrms_norm of member i's slices of U and Y. */
+int cipher_batch_wrms_norm(cipher_batch *b, const double *U_dev, const double *Y_dev,
+                           double atol, double rtol, double *norms);
+
+/* Pick the launch shape (block size, grid cap) of every class's cell lists
+ * by timing candidates on the state X_dev (F_dev is scratch of the same
+ * size): coordinate descent, median of reps batched right-hand sides per
+ * candidate.  Results never depend on the shape.  *ms_out (may be NULL): the
+ * best time of one batched right-hand side. */
+int cipher_batch_autotune(cipher_batch *b, const double *X_dev, double *F_dev, int32_t reps,
+                          double *ms_out);
+/* Report class k's component count, lanes per cell and list q's shape. */
+int cipher_batch_shape(const cipher_batch *b, int32_t k, int32_t q, int32_t *ncp, int32_t *lanes,
+                       int32_t *threads, int32_t *cap);
+
+/* Per-member adaptive integration: cipher_ts_solve for every member at once,
+ * each with its own step size, controller and end time; an attempt advances
+ * all members that have not reached their end time (or max_steps accepted
+ * steps) and is replayed from one captured HIP graph.  t0, t_end, dt_inout
+ * and stats have nmembers entries. */
+int cipher_batch_ts_setup(cipher_batch *b, const cipher_ts_params *p);
+int cipher_batch_ts_solve(cipher_batch *b, double *X_dev, const double *t0, const double *t_end,
+                          double *dt_inout, int64_t max_steps, cipher_ts_stats *stats);
+
 /* Message of the last failing call on this thread. */
 const char *cipher_last_error(void);
 

@@ -375,3 +375,52 @@ def aniso_pair_yaml(res=(16, 16), e_aniso=(0.3, -0.2), m_aniso=(0.5, 0.25), grad
     if gradient:
         txt = txt.replace("interpolation: cubic\n", "interpolation: cubic\n  gradient_calculation: 1\n")
     return txt
+
+
+# small members of a batched ensemble: every component count, all three
+# phase types, lane capacities 4, 8 and 16, uniform and AMR meshes
+ENSEMBLE_SPECS = [
+    {"side": 64, "grains": 12, "components": 1, "types": 1, "capacity": 8, "minlevel": 3, "seed": 11},
+    {"side": 32, "grains": 10, "components": 2, "types": 2, "capacity": 8, "minlevel": 3, "seed": 12},
+    {"side": 32, "grains": 12, "components": 3, "types": 3, "capacity": 8, "minlevel": 3, "seed": 13},
+    {"side": 32, "grains": 10, "components": 4, "types": 3, "capacity": 8, "minlevel": 5, "seed": 14},
+    {"side": 64, "grains": 6, "components": 3, "types": 2, "capacity": 4, "minlevel": 4, "seed": 15},
+    {"side": 64, "grains": 24, "components": 1, "types": 1, "capacity": 16, "minlevel": 6, "seed": 16},
+    {"side": 16, "grains": 4, "components": 2, "types": 1, "capacity": 8, "minlevel": 4, "seed": 17},
+]
+
+
+def ensemble(idx=None):
+    """Workloads of ENSEMBLE_SPECS (cached), or the one at idx."""
+    specs = ENSEMBLE_SPECS if idx is None else [ENSEMBLE_SPECS[idx]]
+    out = []
+    for sp in specs:
+        key = ("ens",) + tuple(sorted(sp.items()))
+        if key not in _cache:
+            _cache[key] = W.ensemble_member(sp)
+        out.append(_cache[key])
+    return out if idx is None else out[0]
+
+
+def ensemble_states(steps=3):
+    """Evolved states of the ensemble members (oracle), cached."""
+    key = ("ens_states", steps)
+    if key not in _cache:
+        _cache[key] = [evolve(wl, initial_state(wl), steps=steps) for wl in ensemble()]
+    return _cache[key]
+
+
+def ulp_spread(wl, X, F, trials=3):
+    """Largest per-field change of the oracle's residual under random one-ulp
+    perturbations of the state (active-set entries kept): the FP64 bar of a
+    state on which the oracle itself is that ill-conditioned."""
+    L = wl.layout
+    D = L["ndof"]
+    rng = np.random.default_rng(0)
+    worst = {}
+    for _ in range(trials):
+        Xp = X * (1.0 + 2.0 ** -52 * rng.choice([-1.0, 1.0], X.size))
+        Xp.reshape(-1, D)[:, :L["as_size"]] = X.reshape(-1, D)[:, :L["as_size"]]
+        for f, e in field_errors(oracle.rhs(wl.cfg, wl.mesh, Xp), F, L).items():
+            worst[f] = max(worst.get(f, 0.0), e)
+    return worst

@@ -62,21 +62,6 @@ for _n in H.REF_CASES:
     WORKLOADS[f"ref_{_n}"] = (lambda n=_n: H.ref_case(n))
 
 
-def _ulp_spread(wl, X, F, trials=3):
-    """Largest per-field change of the oracle's residual under random one-ulp
-    perturbations of the state (active-set entries kept)."""
-    L = wl.layout
-    D = L["ndof"]
-    rng = np.random.default_rng(0)
-    worst = {}
-    for _ in range(trials):
-        Xp = X * (1.0 + 2.0 ** -52 * rng.choice([-1.0, 1.0], X.size))
-        Xp.reshape(-1, D)[:, :L["as_size"]] = X.reshape(-1, D)[:, :L["as_size"]]
-        for f, e in H.field_errors(oracle.rhs(wl.cfg, wl.mesh, Xp), F, L).items():
-            worst[f] = max(worst.get(f, 0.0), e)
-    return worst
-
-
 def _states(wl):
     X0 = H.initial_state(wl)
     X1 = H.evolve(wl, X0, steps=6)
@@ -112,7 +97,7 @@ def test_rhs_fp64_matches_oracle(name):
         ag = _host(act).astype(np.uint32)
         assert np.array_equal(ag, ao), f"{name}/{sname}: active flags differ in {np.count_nonzero(ag != ao)} cells"
         errs = H.field_errors(Fg, Fo, wl.layout)
-        spread = _ulp_spread(wl, X, Fo)
+        spread = H.ulp_spread(wl, X, Fo)
         print("   errors", errs, "oracle ulp spread", spread, flush=True)
         for f, e in errs.items():
             bar = max(RTOL64, 4.0 * spread[f])
