@@ -79,42 +79,16 @@ def measured_traffic(workload):
     return float(m.group(1)) if m else None
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="gg3d")
-    ap.add_argument("--evolve", type=int, default=20)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
-                    help="only run a plain device copy of BYTES bytes (HBM counter calibration)")
-    args = ap.parse_args()
-    if args.copy_probe:
-        copy_probe(args.copy_probe)
-        return
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards by Morton ranges with a one-cell halo; see the design notes)", file=sys.stderr)
-        sys.exit(2)
-
+def prepare_state(ctx, wl, steps):
+    """SetUpProblem state (mu uniform from the matrix composition) whose
+    phase fields are advanced by `steps` explicit steps on the GPU; returns
+    (X, steps done, histogram of active-set sizes)."""
     import torch
     from cipher_amd import abi
     from cipher_amd import workload as W
-    from cipher_amd.runtime import Context
-
-    t0 = time.time()
-    wl, cfgdesc = build_workload(args.workload)
     L = wl.layout
     D = L["ndof"]
     P = L["pf_size"]
-    N, NF = wl.ncells, wl.mesh.nfaces
-    print(f"[bench] workload {wl.name}: {N} cells, {NF} faces, D={D}, built in {time.time() - t0:.1f}s",
-          file=sys.stderr, flush=True)
-
-    ctx = Context(wl.cfg, precision=abi.FP64)
-    ctx.upload_mesh(wl.mesh)
     X = ctx.setup_problem(wl.phase)
     if L["dp_size"]:
         Xh = X.cpu().numpy()
@@ -125,7 +99,7 @@ def main():
     chem = slice(L["dp_offset"], L["tm_offset"])
     good = X.clone()
     evolved = 0
-    for _ in range(args.evolve):
+    for _ in range(steps):
         ctx.rhs(X, F)
         # phase fields only: an explicit step sized for the phase fields is far
         # beyond the stability limit of the stiff chemical-potential dynamics,
@@ -148,6 +122,219 @@ def main():
     del good
     torch.cuda.synchronize()
     nact = np.bincount(np.rint(X.view(-1, D)[:, 0].cpu().numpy()).astype(int), minlength=P + 1)
+    return X, evolved, nact
+
+
+def rel_errors(A, B, L):
+    """Normwise relative error of A against B per field (max|A-B| / max|B|)."""
+    pf = slice(L["pf_offset"], L["pf_offset"] + L["pf_size"])
+    rel = {}
+    for name, sl in (("phase_fraction", pf), ("chemical_potential", slice(L["dp_offset"], L["dp_offset"] + L["dp_size"])),
+                     ("explicit_potential", slice(L["ex_offset"], L["ex_offset"] + L["ex_size"]))):
+        if A[:, sl].size:
+            s = np.abs(B[:, sl]).max()
+            rel[name] = float(np.abs(A[:, sl] - B[:, sl]).max() / s) if s > 0 else float(np.abs(A[:, sl]).max())
+    return rel
+
+
+def run_ensemble(args):
+    """Batched ensemble: `--members` independent 2D instances (side
+    log-uniform 64..1024 with AMR, 10..2000 grains, 1..4 components, 1..3
+    phase types) in one batch; the batched FP32-mixed right-hand side is the
+    measured value, the batched FP64 one its control, and the same members
+    run one context at a time show what batching gains."""
+    import torch
+    from cipher_amd import abi
+    from cipher_amd import workload as W
+    from cipher_amd.runtime import Batch, Context
+
+    t0 = time.time()
+    specs = W.ensemble_specs(n=args.members)
+    wls, ctxs, states = [], [], []
+    nact_tot = np.zeros(17, dtype=np.int64)
+    evolved = []
+    for i, sp in enumerate(specs):
+        wl = W.ensemble_member(sp)
+        ctx = Context(wl.cfg, precision=abi.FP64)
+        ctx.upload_mesh(wl.mesh)
+        X, ev, nact = prepare_state(ctx, wl, args.evolve)
+        nact_tot[:nact.size] += nact
+        evolved.append(ev)
+        wls.append(wl)
+        ctxs.append(ctx)
+        states.append(X)
+        if i % 8 == 7 or i == len(specs) - 1:
+            print(f"[bench] {i + 1}/{len(specs)} members ready ({time.time() - t0:.0f}s)", file=sys.stderr, flush=True)
+    X = torch.cat(states)
+    del states
+    b = Batch(ctxs)
+    F = torch.empty_like(X)
+    N = sum(w.ncells for w in wls)
+    bytes_alg = sum(algorithmic_bytes(w.ncells, w.mesh.nfaces, w.layout["ndof"]) for w in wls)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, steps):
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(steps):
+            fn()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / steps
+
+    res = {}
+    for prec in (abi.FP32_MIXED, abi.FP64):
+        b.set_precision(prec)
+        tune_ms, shapes = b.autotune(X, reps=3)
+        for _ in range(args.warmup):
+            b.rhs(X, F)
+        ms = timed(lambda: b.rhs(X, F), args.steps)
+        res[prec] = (ms, F.clone(), shapes)
+        print(f"[bench] batched {'fp32-mixed' if prec else 'fp64'}: {ms:.3f} ms/RHS "
+              f"(autotuned {tune_ms:.3f})", file=sys.stderr, flush=True)
+    # the same members one context at a time (FP64, default launch shapes)
+    views = [(c, b.slice(X, i), b.slice(F, i)) for i, c in enumerate(ctxs)]
+    for c in ctxs:
+        c.set_precision(abi.FP64)
+    for _ in range(max(1, args.warmup // 4)):
+        for c, x, f in views:
+            c.rhs(x, f)
+    ms_seq = timed(lambda: [c.rhs(x, f) for c, x, f in views], max(1, args.steps // 4))
+    print(f"[bench] one context at a time fp64: {ms_seq:.3f} ms per ensemble RHS", file=sys.stderr, flush=True)
+    ms32, F32, shapes32 = res[abi.FP32_MIXED]
+    ms64, F64, shapes64 = res[abi.FP64]
+    # conditioning of each member's state: the FP64 residual's own change
+    # under random one-ulp perturbations of the state (active sets kept); a
+    # field whose FP64 value is cancellation noise (e.g. the mu rate of a
+    # single-material member at uniform mu, zero in exact arithmetic) cannot
+    # be matched to 1e-6 by any other rounding, so the bar is
+    # max(1e-6, 4 x that spread), as in the parity tests
+    gen = torch.Generator(device=X.device).manual_seed(0)
+    sign = torch.randint(0, 2, X.shape, device=X.device, generator=gen, dtype=torch.int8).to(torch.float64) * 2 - 1
+    Xp = X * (1.0 + 2.0 ** -52 * sign)
+    del sign
+    for i, w in enumerate(wls):
+        D = w.layout["ndof"]
+        nas = w.layout["as_size"]
+        b.slice(Xp, i).view(-1, D)[:, :nas] = b.slice(X, i).view(-1, D)[:, :nas]
+    b.set_precision(abi.FP64)
+    Fp = b.rhs(Xp)
+    A, B, Bp = F32.cpu().numpy(), F64.cpu().numpy(), Fp.cpu().numpy()
+    del Xp, Fp
+    rel, spread, ratio, worst = {}, {}, {}, {}
+    for i, w in enumerate(wls):
+        D = w.layout["ndof"]
+        sl = slice(b.offsets[i], b.offsets[i + 1])
+        e32 = rel_errors(A[sl].reshape(-1, D), B[sl].reshape(-1, D), w.layout)
+        e64 = rel_errors(Bp[sl].reshape(-1, D), B[sl].reshape(-1, D), w.layout)
+        for k, v in e32.items():
+            rel[k] = max(rel.get(k, 0.0), v)
+            spread[k] = max(spread.get(k, 0.0), e64[k])
+            r = v / max(1e-6, 4.0 * e64[k])
+            if r >= ratio.get(k, -1.0):
+                ratio[k] = r
+                worst[k] = f"{w.name}: fp32 {v:.2e}, fp64 spread {e64[k]:.2e}"
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        t_start = time.perf_counter()
+        cells = 0
+        used = 0
+        for i, w in enumerate(wls):
+            oracle.rhs(w.cfg, w.mesh, X[b.offsets[i]:b.offsets[i + 1]].cpu().numpy())
+            cells += w.ncells
+            used += 1
+            if time.perf_counter() - t_start >= args.cpu_seconds:
+                break
+        el = time.perf_counter() - t_start
+        cpu = {"value": cells / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"one oracle RHS of each of the first {used} members ({cells} cells, {el:.1f} s, one thread)"}
+
+    rate32 = N / (ms32 * 1e-3)
+    rate64 = N / (ms64 * 1e-3)
+    gbs32 = bytes_alg / (ms32 * 1e-3) / 1e9
+    gbs64 = bytes_alg / (ms64 * 1e-3) / 1e9
+    comps = [s["components"] for s in specs]
+    out = {
+        "metric": "RHS cell-updates/s + %HBM peak: FP32-mixed vs FP64, <=1e-6 rel err",
+        "value": rate32,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-mixed (f64 state, thermodynamics and active-set decisions)",
+        "data": "synthetic (seeded Voronoi microstructures, reference YAML schema)",
+        "config": {"workload": "batched_ensemble_2d", "members": len(specs), "cells": N,
+                   "sides": [s["side"] for s in specs], "components_hist": np.bincount(comps, minlength=5)[1:].tolist(),
+                   "capacities": sorted(set(s["capacity"] for s in specs)),
+                   "classes": len({(sh[0], sh[1]) for sh in shapes32}),
+                   "evolved_steps": [min(evolved), max(evolved)], "active_set_sizes": nact_tot.tolist(),
+                   "parallelism": "1 GPU, one batch"},
+        "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": bytes_alg},
+        "fp64_control": {"value": rate64, "ms_per_step": ms64, "achieved_gbs": gbs64, "frac": gbs64 / HBM_PEAK_GBS,
+                         "fp32_speedup": ms64 / ms32},
+        "fp32_vs_fp64": {"rel_err_max_over_members": rel, "fp64_one_ulp_spread_max_over_members": spread,
+                         "worst_err_over_bar": ratio, "worst_member": worst,
+                         "phase_fraction_within_1e-6": rel.get("phase_fraction", 0.0) <= 1e-6,
+                         "all_fields_within_bar": all(v <= 1.0 for v in ratio.values())},
+        "one_context_at_a_time_fp64": {"ms_per_step": ms_seq, "batching_speedup_fp64": ms_seq / ms64},
+        "autotuned_shapes_fp32": shapes32,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="gg3d", help="gg3d, gg2d, ternary or ensemble")
+    ap.add_argument("--members", type=int, default=96, help="instances of the ensemble workload")
+    ap.add_argument("--evolve", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the path shards by Morton ranges with a one-cell halo; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    if args.workload == "ensemble":
+        run_ensemble(args)
+        return
+
+    import torch
+    from cipher_amd import abi
+    from cipher_amd import workload as W
+    from cipher_amd.runtime import Context
+
+    t0 = time.time()
+    wl, cfgdesc = build_workload(args.workload)
+    L = wl.layout
+    D = L["ndof"]
+    P = L["pf_size"]
+    N, NF = wl.ncells, wl.mesh.nfaces
+    print(f"[bench] workload {wl.name}: {N} cells, {NF} faces, D={D}, built in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+
+    ctx = Context(wl.cfg, precision=abi.FP64)
+    ctx.upload_mesh(wl.mesh)
+    X, evolved, nact = prepare_state(ctx, wl, args.evolve)
+    F = torch.empty_like(X)
+    pf = slice(L["pf_offset"], L["pf_offset"] + P)
     print(f"[bench] state ready after {evolved} steps: active-set sizes {nact.tolist()}", file=sys.stderr, flush=True)
 
     def time_mode(precision):
@@ -175,13 +362,7 @@ def main():
     ctx.set_precision(abi.FP64)
 
     # FP32-mixed error against FP64, per field (normwise relative)
-    A, B = F32.view(-1, D).cpu().numpy(), F64.view(-1, D).cpu().numpy()
-    rel = {}
-    for name, sl in (("phase_fraction", pf), ("chemical_potential", slice(L["dp_offset"], L["dp_offset"] + L["dp_size"])),
-                     ("explicit_potential", slice(L["ex_offset"], L["ex_offset"] + L["ex_size"]))):
-        if A[:, sl].size:
-            s = np.abs(B[:, sl]).max()
-            rel[name] = float(np.abs(A[:, sl] - B[:, sl]).max() / s) if s > 0 else float(np.abs(A[:, sl]).max())
+    rel = rel_errors(F32.view(-1, D).cpu().numpy(), F64.view(-1, D).cpu().numpy(), L)
 
     bytes_alg = algorithmic_bytes(N, NF, D)
     rate64 = N / (ms64 * 1e-3)

@@ -237,7 +237,7 @@ extern "C" int cipher_ctx_create(const cipher_config *cfg, int precision, cipher
         cipher_ctx_destroy(c);
         return fail(CIPHER_ERR_HIP, "event creation failed");
     }
-    if (hipMalloc(&c->err, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->counts, 8 * sizeof(int32_t)) != hipSuccess || hipMalloc(&c->partial, sizeof(double) * c->npart) != hipSuccess ||
+    if (hipMalloc(&c->err, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->counts, RHS_COUNTERS * sizeof(int32_t)) != hipSuccess || hipMalloc(&c->partial, sizeof(double) * c->npart) != hipSuccess ||
         hipMalloc(&c->norm, sizeof(double)) != hipSuccess ||
         hipHostMalloc(&c->h_norm, sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&c->h_err, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
@@ -391,7 +391,7 @@ extern "C" int cipher_upload_mesh(cipher_ctx *c, const cipher_mesh *m) {
     HIPCHK(hipMalloc(&c->cellgeom, sizeof(double) * std::max<long long>(N, 1)));
     HIPCHK(hipMalloc(&c->cm, cmrec * std::max<long long>(N, 1)));
     HIPCHK(hipMalloc(&c->glist, sizeof(int32_t) * (size_t)(c->L.pf_size + 1) * std::max<long long>(N, 1)));
-    HIPCHK(hipMalloc(&c->list, sizeof(int32_t) * std::max<long long>(N, 1)));
+    HIPCHK(hipMalloc(&c->list, 5 * sizeof(int32_t) * std::max<long long>(N, 1)));
     HIPCHK(hipMalloc(&c->bins, 5 * sizeof(int32_t) * std::max<long long>(N, 1)));
     HIPCHK(hipMemcpy(c->cell_off, off.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->adj, adj.data(), sizeof(int32_t) * adj.size(), hipMemcpyHostToDevice));

@@ -77,7 +77,7 @@ struct cipher_batch {
     std::vector<cipher_ctx *> m;
     std::vector<long long> xoff, celloff;   // nmembers + 1
     BatchMember *d_mem = nullptr;
-    int32_t *d_counts = nullptr;            // 8 per member
+    int32_t *d_counts = nullptr;            // RHS_COUNTERS per member
     int32_t *d_err = nullptr;               // PostStep flag per member
     double *d_partial = nullptr;            // error-norm partial sums
     double *d_norms = nullptr;              // per-member error norms
@@ -187,12 +187,6 @@ static int build_tables(cipher_batch *b) {
             s.lds = 0;
             for (int i : k.members) s.lds = std::max(s.lds, rhs_lds_bytes(b->m[i]->dc, Gq, cpb));
         }
-        if ((rc = make_segs(b, L.redo, k.members,
-                            [&](int i) { return list_blocks(b->m[i]->ncells, Nclass, cpbG, 16384); })))
-            return rc;
-        L.redo.threads = 256;
-        L.redo.lds = 0;
-        for (int i : k.members) L.redo.lds = std::max(L.redo.lds, rhs_lds_bytes(b->m[i]->dc, k.G, cpbG));
     }
     return CIPHER_OK;
 }
@@ -281,7 +275,7 @@ extern "C" int cipher_batch_create(cipher_ctx *const *members, int32_t nmembers,
         npart += wrms_parts(std::max(1LL, c->ncells * (long long)c->L.ndof));
     }
     bool ok = hipMalloc(&b->d_mem, sizeof(BatchMember) * n) == hipSuccess &&
-              hipMalloc(&b->d_counts, sizeof(int32_t) * 8 * n) == hipSuccess &&
+              hipMalloc(&b->d_counts, sizeof(int32_t) * RHS_COUNTERS * n) == hipSuccess &&
               hipMalloc(&b->d_err, sizeof(int32_t) * n) == hipSuccess &&
               hipMalloc(&b->d_partial, sizeof(double) * npart) == hipSuccess &&
               hipMalloc(&b->d_norms, sizeof(double) * n) == hipSuccess &&
@@ -322,7 +316,7 @@ extern "C" int cipher_batch_create(cipher_ctx *const *members, int32_t nmembers,
         m.xoff = b->xoff[i];
         m.celloff = b->celloff[i];
         m.bins = c->bins;
-        m.counts = b->d_counts + 8 * i;
+        m.counts = b->d_counts + RHS_COUNTERS * i;
         m.cm = (double *)c->cm;
         m.glist = c->glist;
         m.err = b->d_err + i;
@@ -393,7 +387,7 @@ extern "C" int cipher_batch_set_stream(cipher_batch *b, void *stream) {
 static int batch_rhs_impl(cipher_batch *b, const double *X, double *F, uint32_t *active, const int32_t *live) {
     hipStream_t st = b->stream;
     int rc;
-    if ((rc = fill(b, b->d_counts, 8LL * (long long)b->m.size(), 0, st))) return rc;
+    if ((rc = fill(b, b->d_counts, (long long)RHS_COUNTERS * (long long)b->m.size(), 0, st))) return rc;
     b->all.X = X;
     b->all.live = live;
     if (launch_fold_batch(b->all, st)) return cipher_fail(CIPHER_ERR_HIP, "batch fold launch");
@@ -417,12 +411,6 @@ static int batch_rhs_impl(cipher_batch *b, const double *X, double *F, uint32_t 
     for (int i = 0; i < 4; i++) {
         BCHK(hipEventRecord(b->ev_join[i], b->aux[i]));
         BCHK(hipStreamWaitEvent(st, b->ev_join[i], 0));
-    }
-    if (b->precision == CIPHER_FP32_MIXED) {
-        for (BatchClass &k : b->cls) {
-            hipStream_t s0[5] = {st, st, st, st, st};
-            if (launch_rhs_batch(k.L, 2, b->precision, s0)) return cipher_fail(CIPHER_ERR_HIP, "batch FP64 redo launch");
-        }
     }
     return CIPHER_OK;
 }

@@ -38,9 +38,9 @@ struct cipher_ctx {
     double *cellgeom = nullptr;
     void *cm = nullptr;
     int32_t *glist = nullptr;
-    int32_t *list = nullptr;        // FP32-mixed recomputation list (ncells)
+    int32_t *list = nullptr;        // FP32-mixed recomputation lists (5 * ncells)
     int32_t *bins = nullptr;        // cells by active-set size (5 * ncells)
-    int32_t *counts = nullptr;      // bin sizes and recomputation count (8)
+    int32_t *counts = nullptr;      // list and recomputation-list sizes (RHS_COUNTERS)
     hipStream_t aux[4] = {nullptr, nullptr, nullptr, nullptr};   // default priority
     hipEvent_t ev_fork = nullptr, ev_join[4] = {nullptr, nullptr, nullptr, nullptr};
     int32_t *err = nullptr;

@@ -23,6 +23,10 @@ struct RhsArgs {
     long long ncells;
 };
 
+// per-mesh list counters: 0..4 the per-size lists, 8..12 their FP32-mixed
+// recomputation lists
+constexpr int RHS_COUNTERS = 16;
+
 // cells per binning tile: 256 threads of BIN_CPT consecutive cells
 constexpr int BIN_CPT = 8;
 constexpr int BIN_TILE = 256 * BIN_CPT;
@@ -42,8 +46,8 @@ struct RhsLaunch {
     const int32_t *cell_off, *adj, *bcell_off, *bface_id;
     void *cm;
     int32_t *bins;            // 5 lists of ncells entries (cells by active-set size)
-    int32_t *counts;          // 8 counters: bin sizes, then the recomputation list
-    int32_t *redo;            // ncells entries
+    int32_t *counts;          // RHS_COUNTERS counters
+    int32_t *redo;            // 5 recomputation lists of ncells entries
     hipStream_t aux[4];       // forked streams for the concurrent lists
     hipEvent_t ev_fork, ev_join[4];
     long long ncells;
@@ -78,7 +82,7 @@ struct BatchMember {
     long long xoff;           // first double of the member in the batch vectors
     long long celloff;        // first cell of the member in batch per-cell arrays
     int32_t *bins;            // 5 lists of a.ncells entries
-    int32_t *counts;          // 8 counters: bin sizes, then the recomputation list
+    int32_t *counts;          // RHS_COUNTERS counters
     double *cm;               // per-cell prep records
     int32_t *glist;           // PostStep scratch, ncells*(P+1)
     int32_t *err;             // PostStep capacity flag of the member
@@ -112,14 +116,14 @@ struct BatchLaunch {
     const BatchMember *mem;
     const int32_t *live;      // per-member flag (null: all members)
     int ncp, G, nbins;
-    BatchSegs fold, prep, bin, list[5], redo, post, dofs;
+    BatchSegs fold, prep, bin, list[5], post, dofs;
     const double *X;
     double *F;
     uint32_t *active;
 };
 
 // phase 0: prep records and per-size lists (on st[0]); phase 1: the lists
-// (list b on st[b]); phase 2: FP32-mixed recomputation list (on st[0])
+// (list b on st[b], in FP32-mixed followed by its FP64 recomputation list)
 cipher_status launch_rhs_batch(const BatchLaunch &L, int phase, int precision, const hipStream_t *st);
 cipher_status launch_fold_batch(const BatchLaunch &L, hipStream_t st);
 cipher_status launch_post_batch(const BatchLaunch &L, double *X, hipStream_t st);

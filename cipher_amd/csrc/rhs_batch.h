@@ -36,8 +36,9 @@ __global__ __launch_bounds__(256) void k_bin_batch(const BatchMember *__restrict
     bin_tile(X + m.xoff, m.a.ncells, c, nbins, m.bins, m.a.ncells, m.counts, (long long)blockIdx.x - seg[k]);
 }
 
-// slot 0..4: the member's list of cells that need 2^slot lanes; 5: its
-// FP32-mixed recomputation list
+// slot 0..4: the member's list of cells that need 2^slot lanes (FP32-mixed
+// appends the cells it leaves to recomputation list slot); slot 5..9: the
+// recomputation list of list slot-5
 template <int G, int GMAX, int NC, class R>
 __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs_batch(const BatchMember *__restrict__ mem,
                                                                      const int32_t *__restrict__ live,
@@ -53,9 +54,11 @@ __global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs_batch(const BatchM
     a.X = X + m.xoff;
     a.F = F + m.xoff;
     a.active = active ? active + m.celloff : nullptr;
-    a.cells = slot < 5 ? m.bins + (long long)slot * a.ncells : m.a.redo;
-    a.cells_count = m.counts + (slot < 5 ? slot : 5);
-    a.redo_count = m.counts + 5;
+    const int q = slot < 5 ? slot : slot - 5;
+    a.redo = m.a.redo + (long long)q * a.ncells;
+    a.redo_count = m.counts + 8 + q;
+    a.cells = slot < 5 ? m.bins + (long long)q * a.ncells : a.redo;
+    a.cells_count = slot < 5 ? m.counts + q : a.redo_count;
     const DevCfg c = m.c;
     rhs_cells<G, GMAX, NC, R>(a, c, (long long)blockIdx.x - seg[k], (long long)(seg[k + 1] - seg[k]));
 }
@@ -67,13 +70,20 @@ static void batch_list(const BatchLaunch &L, int slot, const BatchSegs &s, hipSt
                        s.seg, s.segmem, s.nseg, slot, L.X, L.F, L.active);
 }
 
+// list q, and in FP32-mixed its FP64 recomputation list, on stream st
+template <int G, int GMAX, int NC, class R>
+static void batch_bin(const BatchLaunch &L, int q, hipStream_t st) {
+    batch_list<G, GMAX, NC, R>(L, q, L.list[q], st);
+    if constexpr (sizeof(R) == 4) batch_list<G, GMAX, NC, double>(L, 5 + q, L.list[q], st);
+}
+
 template <int GMAX, int NC, class R>
 static void batch_lists(const BatchLaunch &L, const hipStream_t *st) {
-    if constexpr (GMAX >= 16) batch_list<16, GMAX, NC, R>(L, 4, L.list[4], st[4]);
-    if constexpr (GMAX >= 8) batch_list<8, GMAX, NC, R>(L, 3, L.list[3], st[3]);
-    batch_list<4, GMAX, NC, R>(L, 2, L.list[2], st[2]);
-    batch_list<2, GMAX, NC, R>(L, 1, L.list[1], st[1]);
-    batch_list<1, GMAX, NC, R>(L, 0, L.list[0], st[0]);
+    if constexpr (GMAX >= 16) batch_bin<16, GMAX, NC, R>(L, 4, st[4]);
+    if constexpr (GMAX >= 8) batch_bin<8, GMAX, NC, R>(L, 3, st[3]);
+    batch_bin<4, GMAX, NC, R>(L, 2, st[2]);
+    batch_bin<2, GMAX, NC, R>(L, 1, st[1]);
+    batch_bin<1, GMAX, NC, R>(L, 0, st[0]);
 }
 
 template <int G, int NC>
@@ -88,8 +98,6 @@ static cipher_status batch_G(const BatchLaunch &L, int phase, int precision, con
     } else if (phase == 1) {
         if (precision == CIPHER_FP32_MIXED) batch_lists<G, NC, float>(L, st);
         else batch_lists<G, NC, double>(L, st);
-    } else if (precision == CIPHER_FP32_MIXED) {
-        batch_list<G, G, NC, double>(L, 5, L.redo, st[0]);
     }
     return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
 }

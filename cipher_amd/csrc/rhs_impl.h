@@ -284,7 +284,7 @@ __device__ __forceinline__ void wave_sync() {
 // two-phase cell occupies two lanes, not PF_SIZE).  Each group of G lanes
 // stages its cell's record in LDS, computes, and writes the cell's F record
 // itself.  In FP32-mixed, cells whose active flags fall inside the FP32
-// rounding band are appended to a.redo for the FP64 pass.
+// rounding band are appended to a.redo for an FP64 pass over the same list.
 // (blocks blk, blk + nblk, ... of the list; a block is 256/G groups)
 template <int G, int GMAX, int NC, class R>
 __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, long long blk, long long nblk) {
@@ -338,7 +338,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
         // thermal flux terms), then accumulated into the cell's sums in face
         // order, so every sum keeps the reference's sequence.
         Acc<R> lap;
-        Acc<R> dlap[NDV];
+        Acc<double> dlap[NDV];   // chemical fluxes stay FP64 in both modes
         Acc<R> tlap;
         R lapabs = R(0);
         double cmS[2 * NC + 1];
@@ -381,47 +381,46 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
             const int Lid0 = selfL ? id0 : clamp_id(id0d, c.npf);
             const int lguard = (fon && need_lap && (Ln > 1 || Lid0 > 1)) ? 1 : 0;
             const int guard = (lguard || (fon && grad_on)) ? 1 : 0;
-            R cterm[NDV];
+            // chemical flux (src/cipher.c:290-308) in FP64 in both modes: the
+            // divergence of nearly balanced fluxes loses digits that an FP32
+            // flux would not have
+            double cterm[NDV];
 #pragma unroll
-            for (int q = 0; q < NDV; q++) cterm[q] = R(0);
+            for (int q = 0; q < NDV; q++) cterm[q] = 0.0;
             if constexpr (NC > 1) {
                 if (fon) {
                     double cmN[2 * NC + 1];
 #pragma unroll
                     for (int q = 0; q < 2 * NC + 1; q++) cmN[q] = a.cm[(long long)nbf * (2 * NC + 1) + q];
-                    R me[NC], ca[NC], MB[ND * ND], dmu[ND], fd[ND];
+                    double me[NC], ca[NC], MB[ND * ND], dmu[ND], fd[ND];
 #pragma unroll
                     for (int q = 0; q < NC; q++) {
-                        const double mLd = selfL ? cmS[NC + q] : cmN[NC + q];
-                        const double mRd = selfL ? cmN[NC + q] : cmS[NC + q];
-                        const R mL = (R)mLd, mR = (R)mRd;
-                        if (fabs(mLd) > 1e-32 && fabs(mRd) > 1e-32) {
-                            if constexpr (MIXED) me[q] = R(2) * mL * (mR / (mL + mR));   // no FP32 underflow
-                            else me[q] = R(2) * mL * mR / (mL + mR);
-                        } else {
-                            me[q] = R(0);
-                        }
-                        const R cL = (R)(selfL ? cmS[q] : cmN[q]), cR = (R)(selfL ? cmN[q] : cmS[q]);
-                        ca[q] = R(2) * cL * cR / (cL + cR);
+                        const double mL = selfL ? cmS[NC + q] : cmN[NC + q];
+                        const double mR = selfL ? cmN[NC + q] : cmS[NC + q];
+                        if (fabs(mL) > 1e-32 && fabs(mR) > 1e-32) me[q] = 2.0 * mL * mR / (mL + mR);
+                        else me[q] = 0.0;
+                        const double cL = selfL ? cmS[q] : cmN[q], cR = selfL ? cmN[q] : cmS[q];
+                        ca[q] = 2.0 * cL * cR / (cL + cR);
                     }
 #pragma unroll
                     for (int i = 0; i < ND; i++)
 #pragma unroll
                         for (int jj = 0; jj < ND; jj++) {
-                            R v = R(0);
+                            double v = 0.0;
 #pragma unroll
                             for (int q = 0; q < NC; q++)
-                                v += ((jj == q ? R(1) : R(0)) - ca[jj]) * ((q == i ? R(1) : R(0)) - ca[i]) * me[q];
+                                v += ((jj == q ? 1.0 : 0.0) - ca[jj]) * ((q == i ? 1.0 : 0.0) - ca[i]) * me[q];
                             MB[i * ND + jj] = v;
                         }
 #pragma unroll
                     for (int q = 0; q < ND; q++) {
                         const double mus = x[c.dpo + q], mun = xf[c.dpo + q];
-                        dmu[q] = selfL ? (R)(mun - mus) : (R)(mus - mun);
+                        dmu[q] = selfL ? (mun - mus) : (mus - mun);
                     }
-                    dmatvec<ND, R>(fd, MB, dmu);
+                    dmatvec<ND, double>(fd, MB, dmu);
+                    const double vold = selfL ? vL : vR;
 #pragma unroll
-                    for (int q = 0; q < ND; q++) cterm[q] = selfL ? ff * fd[q] / vol : -(ff * fd[q] / vol);
+                    for (int q = 0; q < ND; q++) cterm[q] = selfL ? ffd * fd[q] / vold : -(ffd * fd[q] / vold);
                 }
             }
             R tterm = R(0);
@@ -533,15 +532,15 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                 for (int k = bb; k < be; k++) {
                     if (a.bface_id[k] != bc.boundary_id) continue;
                     const R dL = (R)hS;
-                    R fd[NDV];
+                    double fd[NDV];
                     R ft = R(0);
                     if (bc.chem_type != CIPHER_BC_NONE) {
 #pragma unroll
                         for (int q = 0; q < ND; q++) {
-                            fd[q] = R(0);
+                            fd[q] = 0.0;
                             if (bc.chem_mask & (1 << q)) {
-                                if (bc.chem_type == CIPHER_BC_NEUMANN) fd[q] = (R)bc.chem_val[q] * dL;
-                                else fd[q] = (R)cmS[NC + q] * ((R)bc.chem_val[q] - (R)x[c.dpo + q]);
+                                if (bc.chem_type == CIPHER_BC_NEUMANN) fd[q] = bc.chem_val[q] * hS;
+                                else fd[q] = cmS[NC + q] * (bc.chem_val[q] - x[c.dpo + q]);
                             }
                         }
                     }
@@ -550,8 +549,9 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                     else if (bc.thermal_type == CIPHER_BC_DIRICHLET) ft = tc * ((R)bc.thermal_val - T);
                     const R cf = R(1) / dfastpow(dL, 2u);
                     if (bc.chem_type != CIPHER_BC_NONE) {
+                        const double cfd = 1.0 / dfastpow(hS, 2u);
 #pragma unroll
-                        for (int q = 0; q < ND; q++) dlap[q].add(cf * fd[q]);
+                        for (int q = 0; q < ND; q++) dlap[q].add(cfd * fd[q]);
                     }
                     if (bc.thermal_type != CIPHER_BC_NONE) tlap.add(cf * ft);
                 }
@@ -1118,27 +1118,37 @@ static void launch_list(const RhsArgs &a0, const DevCfg &c, const int32_t *cells
     hipLaunchKernelGGL((k_rhs<G, GMAX, NC, R>), dim3((unsigned)nb), dim3(T), rhs_lds_bytes(c, G, cpb), st, a, c);
 }
 
+// One list q (cells of 2^q lanes); in FP32-mixed followed, on the same
+// stream, by the FP64 recomputation of the list's cells that the FP32 pass
+// left (counts[8+q] cells at redo + q*N).
+template <int G, int GMAX, int NC, class R>
+static void launch_bin(const RhsArgs &a0, const DevCfg &c, const RhsLaunch &L, int q, hipStream_t st) {
+    const long long N = L.ncells;
+    RhsArgs a = a0;
+    a.redo = L.redo + q * N;
+    a.redo_count = L.counts + 8 + q;
+    launch_list<G, GMAX, NC, R>(a, c, L.bins + q * N, L.counts + q, N, st);
+    if constexpr (sizeof(R) == 4) launch_list<G, GMAX, NC, double>(a, c, a.redo, a.redo_count, N, st);
+}
+
 // The lists run concurrently: the heaviest (largest active sets, longest
 // per-cell latency) on the caller's stream, the others on forked streams, so
 // one list's tail overlaps the others' work.
 template <int GMAX, int NC, class R>
 static bool launch_bins(const RhsArgs &a, const DevCfg &c, const RhsLaunch &L, hipStream_t st) {
-    int32_t *bins = L.bins;
-    int32_t *counts = L.counts;
-    const long long N = L.ncells;
     bool ok = hipEventRecord(L.ev_fork, st) == hipSuccess;
     const hipStream_t *s = L.aux;
     for (int i = 0; i < 4; i++) ok = ok && hipStreamWaitEvent(s[i], L.ev_fork, 0) == hipSuccess;
     if (!ok) return false;
     if constexpr (GMAX >= 16) {
-        launch_list<16, GMAX, NC, R>(a, c, bins + 4 * N, counts + 4, N, st);
-        launch_list<8, GMAX, NC, R>(a, c, bins + 3 * N, counts + 3, N, s[3]);
+        launch_bin<16, GMAX, NC, R>(a, c, L, 4, st);
+        launch_bin<8, GMAX, NC, R>(a, c, L, 3, s[3]);
     } else if constexpr (GMAX >= 8) {
-        launch_list<8, GMAX, NC, R>(a, c, bins + 3 * N, counts + 3, N, st);
+        launch_bin<8, GMAX, NC, R>(a, c, L, 3, st);
     }
-    launch_list<4, GMAX, NC, R>(a, c, bins + 2 * N, counts + 2, N, GMAX >= 8 ? s[2] : st);
-    launch_list<2, GMAX, NC, R>(a, c, bins + N, counts + 1, N, s[1]);
-    launch_list<1, GMAX, NC, R>(a, c, bins, counts, N, s[0]);
+    launch_bin<4, GMAX, NC, R>(a, c, L, 2, GMAX >= 8 ? s[2] : st);
+    launch_bin<2, GMAX, NC, R>(a, c, L, 1, s[1]);
+    launch_bin<1, GMAX, NC, R>(a, c, L, 0, s[0]);
     ok = hipGetLastError() == hipSuccess;
     for (int i = 0; i < 4; i++) {
         ok = ok && hipEventRecord(L.ev_join[i], s[i]) == hipSuccess;
@@ -1165,16 +1175,14 @@ static cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st)
     a.cell_off = L.cell_off; a.adj = L.adj; a.bcell_off = L.bcell_off; a.bface_id = L.bface_id;
     a.cm = (const double *)L.cm; a.ncells = N;
     a.cells = nullptr; a.cells_count = nullptr;
-    a.redo = L.redo; a.redo_count = L.counts + 5;
-    // counts[0..4]: bin sizes, counts[5]: FP32-mixed recomputation list
-    hipLaunchKernelGGL(k_fill_i32<0>, dim3(1), dim3(64), 0, st, L.counts, 8, 0);
+    a.redo = nullptr; a.redo_count = nullptr;
+    // counts[0..4]: list sizes, counts[8..12]: their FP32-mixed recomputation lists
+    hipLaunchKernelGGL(k_fill_i32<0>, dim3(1), dim3(64), 0, st, L.counts, RHS_COUNTERS, 0);
     const int nbins = G >= 16 ? 5 : (G >= 8 ? 4 : 3);
     hipLaunchKernelGGL(k_bin<0>, dim3((unsigned)((N + BIN_TILE - 1) / BIN_TILE)), dim3(256), 0, st, L.X, N, c, nbins,
                        L.bins, N, L.counts);
     if (precision == CIPHER_FP32_MIXED) {
         if (!launch_bins<G, NC, float>(a, c, L, st)) return CIPHER_ERR_HIP;
-        // FP64 recomputation of the cells whose flags sat in the FP32 band
-        launch_list<G, G, NC, double>(a, c, L.redo, L.counts + 5, N, st);
     } else {
         if (!launch_bins<G, NC, double>(a, c, L, st)) return CIPHER_ERR_HIP;
     }

@@ -296,7 +296,9 @@ def ensemble_specs(n=96, seed=SEED, min_side=64, max_side=1024, max_grains=2000)
     in [min_side, max_side]; AMR from three levels below the finest),
     grains log-uniform in [10, max_grains] (at most one per 64 voxels),
     components uniform in 1..4, phase types uniform in 1..3 (1 without
-    chemistry), capacity 8 (every eighth instance 4, every twelfth 16)."""
+    chemistry), capacity 8 (every eighth instance 4, with at most one grain
+    per 64x64 voxels so that no union of neighbouring sets exceeds 4; every
+    twelfth 16)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     lo, hi = int(round(math.log2(min_side))), int(round(math.log2(max_side)))
     out = []
@@ -308,6 +310,8 @@ def ensemble_specs(n=96, seed=SEED, min_side=64, max_side=1024, max_grains=2000)
         nc = int(rng.integers(1, 5))
         nt = int(rng.integers(1, 4)) if nc > 1 else 1
         cap = 4 if i % 8 == 7 else (16 if i % 12 == 11 else 8)
+        if cap == 4:
+            g = max(10, min(g, side * side // 4096))
         out.append({"side": side, "grains": g, "components": nc, "types": nt, "capacity": cap,
                     "minlevel": max(k - 3, 3), "seed": seed + 7919 * (i + 1)})
     return out
