@@ -186,7 +186,7 @@ def run_ensemble(args):
     res = {}
     for prec in (abi.FP32_MIXED, abi.FP64):
         b.set_precision(prec)
-        tune_ms, shapes = b.autotune(X, reps=3)
+        tune_ms, shapes = b.autotune(X, reps=3) if not args.no_tune else (0.0, b.shapes())
         for _ in range(args.warmup):
             b.rhs(X, F)
         ms = timed(lambda: b.rhs(X, F), args.steps)
@@ -301,6 +301,7 @@ def main():
     ap.add_argument("--evolve", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of autotuned ones")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -337,8 +338,14 @@ def main():
     pf = slice(L["pf_offset"], L["pf_offset"] + P)
     print(f"[bench] state ready after {evolved} steps: active-set sizes {nact.tolist()}", file=sys.stderr, flush=True)
 
+    shapes = {}
+
     def time_mode(precision):
         ctx.set_precision(precision)
+        if not args.no_tune:
+            # launch shapes picked by the library's autotuner on this state
+            # (untimed set-up, as a user would run it once per mesh)
+            shapes[precision] = ctx.autotune(X, reps=5)[1]
         for _ in range(args.warmup):
             ctx.rhs(X, F)
         torch.cuda.synchronize()
@@ -405,6 +412,8 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_alg},
         "fp32_mixed": {"value": rate32, "ms_per_step": ms32, "achieved_gbs": gbs32,
                        "frac": gbs32 / HBM_PEAK_GBS, "speedup_vs_fp64": ms64 / ms32, "rel_err_vs_fp64": rel},
+        "list_shapes": {"autotuned": not args.no_tune,
+                        "fp64": shapes.get(abi.FP64), "fp32_mixed": shapes.get(abi.FP32_MIXED)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

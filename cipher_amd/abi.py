@@ -23,6 +23,7 @@ STATUS = {0: "CIPHER_OK", 1: "CIPHER_ERR_ARG", 2: "CIPHER_ERR_HIP",
           3: "CIPHER_ERR_CAPACITY", 4: "CIPHER_ERR_NONFINITE",
           5: "CIPHER_ERR_UNSUPPORTED", 6: "CIPHER_ERR_STATE"}
 ERR_CAPACITY = 3
+ERR_ARG = 1
 
 
 class TSeries(C.Structure):
@@ -118,6 +119,13 @@ class TsStats(C.Structure):
                 ("dt", C.c_double), ("err", C.c_double)]
 
 
+class ListShape(C.Structure):
+    _fields_ = [("lanes", C.c_int32), ("cap", C.c_int32), ("map", C.c_int32), ("pad_", C.c_int32)]
+
+    def tuple(self):
+        return (self.lanes, self.cap, self.map)
+
+
 def ts_params(atol, rtol, dt_min, dt_max, safety=0.9, reject_safety=0.5, clip=(0.1, 10.0), use_graph=False):
     return TsParams(atol, rtol, dt_min, dt_max, safety, reject_safety, clip[0], clip[1], 1 if use_graph else 0, 0)
 
@@ -161,7 +169,11 @@ def _declare(lib):
         "cipher_batch_poststep": (i32, [vp, vp, C.POINTER(C.c_int32)]),
         "cipher_batch_wrms_norm": (i32, [vp, vp, vp, dbl, dbl, C.POINTER(C.c_double)]),
         "cipher_batch_autotune": (i32, [vp, vp, vp, i32, C.POINTER(C.c_double)]),
-        "cipher_batch_shape": (i32, [vp, i32, i32] + [C.POINTER(C.c_int32)] * 4),
+        "cipher_batch_shape": (i32, [vp, i32, i32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(ListShape)]),
+        "cipher_get_shape": (i32, [vp, i32, C.POINTER(ListShape)]),
+        "cipher_set_shape": (i32, [vp, i32, C.POINTER(ListShape)]),
+        "cipher_autotune": (i32, [vp, vp, vp, i32, C.POINTER(C.c_double)]),
         "cipher_batch_ts_setup": (i32, [vp, C.POINTER(TsParams)]),
         "cipher_batch_ts_solve": (i32, [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                         C.POINTER(C.c_double), i64, C.POINTER(TsStats)]),
@@ -181,7 +193,7 @@ EXPORTED = ("cipher_layout_of", "cipher_ctx_create", "cipher_ctx_destroy", "ciph
             "cipher_batch_create", "cipher_batch_destroy", "cipher_batch_offsets", "cipher_batch_set_precision",
             "cipher_batch_set_stream", "cipher_batch_rhs", "cipher_batch_rhs_active", "cipher_batch_poststep",
             "cipher_batch_wrms_norm", "cipher_batch_autotune", "cipher_batch_shape", "cipher_batch_ts_setup",
-            "cipher_batch_ts_solve")
+            "cipher_batch_ts_solve", "cipher_get_shape", "cipher_set_shape", "cipher_autotune")
 
 
 def load_library(path=None):

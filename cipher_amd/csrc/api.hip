@@ -7,6 +7,7 @@
 // No entry point allocates after cipher_upload_mesh.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -226,17 +227,6 @@ extern "C" int cipher_ctx_create(const cipher_config *cfg, int precision, cipher
         HIPCHK(hipMemcpy(c->tsv, nanv.data(), sizeof(double) * nanv.size(), hipMemcpyHostToDevice));
     }
     if (c->fold) d.tsv = c->tsv;
-    for (int i = 0; i < 4; i++) {
-        if (hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming) != hipSuccess) {
-            cipher_ctx_destroy(c);
-            return fail(CIPHER_ERR_HIP, "stream creation failed");
-        }
-    }
-    if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess) {
-        cipher_ctx_destroy(c);
-        return fail(CIPHER_ERR_HIP, "event creation failed");
-    }
     if (hipMalloc(&c->err, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->counts, RHS_COUNTERS * sizeof(int32_t)) != hipSuccess || hipMalloc(&c->partial, sizeof(double) * c->npart) != hipSuccess ||
         hipMalloc(&c->norm, sizeof(double)) != hipSuccess ||
         hipHostMalloc(&c->h_norm, sizeof(double), hipHostMallocDefault) != hipSuccess ||
@@ -257,11 +247,6 @@ extern "C" int cipher_ctx_destroy(cipher_ctx *c) {
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->err) (void)hipFree(c->err);
     if (c->counts) (void)hipFree(c->counts);
-    for (int i = 0; i < 4; i++) {
-        if (c->aux[i]) (void)hipStreamDestroy(c->aux[i]);
-        if (c->ev_join[i]) (void)hipEventDestroy(c->ev_join[i]);
-    }
-    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->tsv) (void)hipFree(c->tsv);
     if (c->partial) (void)hipFree(c->partial);
     if (c->norm) (void)hipFree(c->norm);
@@ -433,12 +418,8 @@ int cipher_rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active)
     L.redo = c->list;
     L.bins = c->bins;
     L.counts = c->counts;
-    for (int i = 0; i < 4; i++) {
-        L.aux[i] = c->aux[i];
-        L.ev_join[i] = c->ev_join[i];
-    }
-    L.ev_fork = c->ev_fork;
     L.ncells = c->ncells;
+    for (int q = 0; q < 5; q++) L.shape[q] = c->shape[q];
     if (c->fold) {
         cipher_status f = launch_tsfold(X, c->ncells, c->dc, c->tsv, c->stream);
         if (f != CIPHER_OK) return fail(f, "fold launch");
@@ -457,6 +438,82 @@ extern "C" int cipher_rhs_active(cipher_ctx *c, double t, const double *X, doubl
     (void)t;
     if (!active) return fail(CIPHER_ERR_ARG, "null active buffer");
     return cipher_rhs_impl(c, X, F, active);
+}
+
+static int lanes_cap(const cipher_ctx *c) {
+    const int P = c->L.pf_size;
+    return P <= 4 ? 4 : (P <= 8 ? 8 : 16);
+}
+
+extern "C" int cipher_get_shape(const cipher_ctx *c, int32_t q, cipher_list_shape *out) {
+    if (!c || !out || q < 0 || q >= 5) return fail(CIPHER_ERR_ARG, "bad argument");
+    const ListShape &s = c->shape[q];
+    out->lanes = s.lanes > 0 ? s.lanes : (1 << q);
+    out->cap = s.cap;
+    out->map = s.map;
+    out->pad_ = 0;
+    return CIPHER_OK;
+}
+
+extern "C" int cipher_set_shape(cipher_ctx *c, int32_t q, const cipher_list_shape *shape) {
+    if (!c || !shape || q < 0 || q >= 5) return fail(CIPHER_ERR_ARG, "bad argument");
+    if (!shape_valid(*shape, q, lanes_cap(c))) return fail(CIPHER_ERR_ARG, "list shape out of range");
+    ListShape &s = c->shape[q];
+    s.lanes = shape->lanes;
+    s.cap = shape->cap;
+    s.map = shape->map;
+    cipher_ts_invalidate(c);
+    return CIPHER_OK;
+}
+
+// Coordinate descent over each list's shape (lanes, block order, grid cap), every candidate timed as the median of reps right-hand sides
+// on X in the context's precision; the fastest is kept.
+extern "C" int cipher_autotune(cipher_ctx *c, const double *X, double *F, int32_t reps, double *ms_out) {
+    if (!c || !X || !F) return fail(CIPHER_ERR_ARG, "null argument");
+    if (!c->cell_off) return fail(CIPHER_ERR_STATE, "no mesh uploaded");
+    if (reps < 1) reps = 1;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = CIPHER_OK;
+    auto time_once = [&](float &ms) -> int {
+        std::vector<float> v;
+        for (int r = 0; r < reps; r++) {
+            HIPCHK(hipEventRecord(e0, c->stream));
+            int q = cipher_rhs_impl(c, X, F, nullptr);
+            if (q) return q;
+            HIPCHK(hipEventRecord(e1, c->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, e0, e1));
+            v.push_back(t);
+        }
+        std::sort(v.begin(), v.end());
+        ms = v[v.size() / 2];
+        return CIPHER_OK;
+    };
+    float best = 0.f;
+    if ((rc = cipher_rhs_impl(c, X, F, nullptr)) == CIPHER_OK) rc = time_once(best);
+    const int G = lanes_cap(c);
+    const int nbins = G >= 16 ? 5 : (G >= 8 ? 4 : 3);
+    for (int q = 0; q < nbins && rc == CIPHER_OK; q++) {
+        for (const ShapeStep &step : shape_steps(q, G, c->dc.ncp)) {
+            const ListShape keep = c->shape[q];
+            const ListShape cand = apply_step(keep, step);
+            const int kl = keep.lanes > 0 ? keep.lanes : (1 << q), cl = cand.lanes > 0 ? cand.lanes : (1 << q);
+            if (kl == cl && cand.map == keep.map && cand.cap == keep.cap) continue;
+            c->shape[q] = cand;
+            float ms = 0.f;
+            if ((rc = time_once(ms))) break;
+            if (ms < best) best = ms;
+            else c->shape[q] = keep;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    cipher_ts_invalidate(c);
+    if (rc == CIPHER_OK && ms_out) *ms_out = best;
+    return rc;
 }
 
 int cipher_post_common(cipher_ctx *c, double *X, const int32_t *phase, bool setup) {

@@ -16,8 +16,9 @@
 // (step sizes, live flags) sit in device memory, so it is captured once in
 // a HIP graph and replayed.
 //
-// Launch shapes (block size and grid cap of each class's cell lists) come
-// from cipher_batch_autotune, which times candidates on a given state.
+// Launch shapes (lanes per cell, grid cap and block order of each class's
+// cell lists) come from cipher_batch_autotune, which times candidates on a
+// given state.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,12 +43,6 @@ static const double B[3] = {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0};
 static const double BH[4] = {7.0 / 24.0, 1.0 / 4.0, 1.0 / 3.0, 1.0 / 8.0};
 
 namespace {
-
-// launch shape of one cell list of a class
-struct ListShape {
-    int threads = 256;
-    int cap = 16384;    // blocks of the whole launch, shared out by cell count
-};
 
 struct BatchClass {
     int ncp = 1, G = 8;
@@ -129,6 +124,39 @@ static int make_segs(cipher_batch *b, BatchSegs &s, const std::vector<int> &memb
     return CIPHER_OK;
 }
 
+// segment table of a merged list launch: for each list q in `order`, every
+// member i gets nb(i, q) blocks
+template <class F>
+static int make_list_segs(cipher_batch *b, BatchSegs &s, const std::vector<int> &members, const std::vector<int> &order,
+                          F nb) {
+    std::vector<int32_t> seg{0}, segmem, segq;
+    long long tot = 0;
+    for (int q : order)
+        for (int i : members) {
+            const long long n = nb(i, q);
+            if (n <= 0) continue;
+            tot += n;
+            if (tot > 0x7fffffffLL) return cipher_fail(CIPHER_ERR_UNSUPPORTED, "batch launch exceeds 2^31 blocks");
+            seg.push_back((int32_t)tot);
+            segmem.push_back(i);
+            segq.push_back(q);
+        }
+    s.nseg = (int)segmem.size();
+    s.nblocks = (int)tot;
+    s.seg = s.segmem = s.segq = nullptr;
+    if (s.nseg == 0) return CIPHER_OK;
+    int32_t *d = nullptr;
+    BCHK(hipMalloc(&d, sizeof(int32_t) * (seg.size() + 2 * segmem.size())));
+    b->segs.push_back(d);
+    BCHK(hipMemcpy(d, seg.data(), sizeof(int32_t) * seg.size(), hipMemcpyHostToDevice));
+    BCHK(hipMemcpy(d + seg.size(), segmem.data(), sizeof(int32_t) * segmem.size(), hipMemcpyHostToDevice));
+    BCHK(hipMemcpy(d + seg.size() + segmem.size(), segq.data(), sizeof(int32_t) * segq.size(), hipMemcpyHostToDevice));
+    s.seg = d;
+    s.segmem = d + seg.size();
+    s.segq = d + seg.size() + segmem.size();
+    return CIPHER_OK;
+}
+
 static long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
 
 // blocks of member i in a grid-striding list launch: enough for every cell
@@ -171,22 +199,23 @@ static int build_tables(cipher_batch *b) {
         for (int i : k.members) L.prep.lds = std::max(L.prep.lds, (size_t)cpbG * b->m[i]->L.ndof * sizeof(double));
         if ((rc = make_segs(b, L.bin, k.members, [&](int i) { return cdiv(b->m[i]->ncells, BIN_TILE); }))) return rc;
         if ((rc = make_segs(b, L.post, k.members, [&](int i) { return cdiv(b->m[i]->ncells, cpbG); }))) return rc;
+        // all lists of all members in one launch, largest active sets first
+        std::vector<int> order;
+        L.lists.lds = 0;
         for (int q = 0; q < 5; q++) {
-            BatchSegs &s = L.list[q];
-            const int Gq = 1 << q;
-            if (q >= L.nbins) {
-                s = BatchSegs{};
-                continue;
-            }
-            const int T = k.shape[q].threads;
-            const int cpb = T / Gq;
-            if ((rc = make_segs(b, s, k.members,
-                                [&](int i) { return list_blocks(b->m[i]->ncells, Nclass, cpb, k.shape[q].cap); })))
-                return rc;
-            s.threads = T;
-            s.lds = 0;
-            for (int i : k.members) s.lds = std::max(s.lds, rhs_lds_bytes(b->m[i]->dc, Gq, cpb));
+            const int lanes = k.shape[q].lanes > 0 ? k.shape[q].lanes : (1 << q);
+            L.shape.lanes[q] = lanes;
+            L.shape.map[q] = k.shape[q].map;
         }
+        for (int q = L.nbins - 1; q >= 0; q--) {
+            order.push_back(q);
+            for (int i : k.members)
+                L.lists.lds = std::max(L.lists.lds, rhs_lds_bytes(b->m[i]->dc, L.shape.lanes[q], 256 / L.shape.lanes[q]));
+        }
+        if ((rc = make_list_segs(b, L.lists, k.members, order, [&](int i, int q) {
+                 return list_blocks(b->m[i]->ncells, Nclass, 256 / L.shape.lanes[q], k.shape[q].cap);
+             })))
+            return rc;
     }
     return CIPHER_OK;
 }
@@ -391,26 +420,27 @@ static int batch_rhs_impl(cipher_batch *b, const double *X, double *F, uint32_t 
     b->all.X = X;
     b->all.live = live;
     if (launch_fold_batch(b->all, st)) return cipher_fail(CIPHER_ERR_HIP, "batch fold launch");
-    for (BatchClass &k : b->cls) {
+    // classes are independent: they run on the caller's stream and four
+    // forked streams, each class's launches in order on its stream
+    const bool fork = b->cls.size() > 1;
+    if (fork) {
+        BCHK(hipEventRecord(b->ev_fork, st));
+        for (int i = 0; i < 4; i++) BCHK(hipStreamWaitEvent(b->aux[i], b->ev_fork, 0));
+    }
+    for (size_t ki = 0; ki < b->cls.size(); ki++) {
+        BatchClass &k = b->cls[ki];
         k.L.X = X;
         k.L.F = F;
         k.L.active = active;
         k.L.live = live;
-        hipStream_t s0[5] = {st, st, st, st, st};
-        if (launch_rhs_batch(k.L, 0, b->precision, s0)) return cipher_fail(CIPHER_ERR_HIP, "batch prep/bin launch");
+        hipStream_t s = (ki % 5 == 0) ? st : b->aux[ki % 5 - 1];
+        if (launch_rhs_batch(k.L, b->precision, s)) return cipher_fail(CIPHER_ERR_HIP, "batch list launch");
     }
-    BCHK(hipEventRecord(b->ev_fork, st));
-    for (int i = 0; i < 4; i++) BCHK(hipStreamWaitEvent(b->aux[i], b->ev_fork, 0));
-    for (BatchClass &k : b->cls) {
-        // the list of the largest active sets on the caller's stream, the
-        // others on forked streams
-        hipStream_t s[5];
-        for (int q = 0; q < 5; q++) s[q] = (q == k.L.nbins - 1) ? st : b->aux[q < 4 ? q : 3];
-        if (launch_rhs_batch(k.L, 1, b->precision, s)) return cipher_fail(CIPHER_ERR_HIP, "batch list launch");
-    }
-    for (int i = 0; i < 4; i++) {
-        BCHK(hipEventRecord(b->ev_join[i], b->aux[i]));
-        BCHK(hipStreamWaitEvent(st, b->ev_join[i], 0));
+    if (fork) {
+        for (int i = 0; i < 4; i++) {
+            BCHK(hipEventRecord(b->ev_join[i], b->aux[i]));
+            BCHK(hipStreamWaitEvent(st, b->ev_join[i], 0));
+        }
     }
     return CIPHER_OK;
 }
@@ -469,9 +499,10 @@ extern "C" int cipher_batch_wrms_norm(cipher_batch *b, const double *U, const do
 
 // ---- autotuner -------------------------------------------------------------
 // Coordinate descent over the launch shape of each class's lists: for every
-// (class, list) in turn, each candidate block size and grid cap is timed as
-// the median of `reps` whole batched right-hand sides on X, and the fastest
-// is kept.  The choice changes only launch shapes, never results.
+// (class, list) in turn, lanes per cell, block order and grid cap are varied
+// one at a time, each candidate timed as the median of `reps`
+// whole batched right-hand sides on X, and the fastest kept.  The choice
+// changes only launch shapes, never results.
 extern "C" int cipher_batch_autotune(cipher_batch *b, const double *X, double *F, int32_t reps, double *ms_out) {
     if (!b || !X || !F) return cipher_fail(CIPHER_ERR_ARG, "null argument");
     if (reps < 1) reps = 1;
@@ -498,28 +529,24 @@ extern "C" int cipher_batch_autotune(cipher_batch *b, const double *X, double *F
     };
     float best = 0.f;
     if ((rc = batch_rhs_impl(b, X, F, nullptr, nullptr)) == CIPHER_OK) rc = time_once(best);
-    static const int TH[3] = {64, 128, 256};
-    static const int CAP[3] = {4096, 16384, 65536};
     for (size_t ki = 0; ki < b->cls.size() && rc == CIPHER_OK; ki++) {
         BatchClass &k = b->cls[ki];
         for (int q = 0; q < k.L.nbins && rc == CIPHER_OK; q++) {
-            const ListShape keep = k.shape[q];
-            ListShape pick = keep;
-            for (int ti = 0; ti < 3 && rc == CIPHER_OK; ti++)
-                for (int ci = 0; ci < 3 && rc == CIPHER_OK; ci++) {
-                    if (TH[ti] < (1 << q)) continue;
-                    ListShape cand{TH[ti], CAP[ci]};
-                    if (cand.threads == keep.threads && cand.cap == keep.cap) continue;
-                    k.shape[q] = cand;
-                    if ((rc = build_tables(b))) break;
-                    float ms = 0.f;
-                    if ((rc = time_once(ms))) break;
-                    if (ms < best) {
-                        best = ms;
-                        pick = cand;
-                    }
+            for (const ShapeStep &step : shape_steps(q, k.G, k.ncp)) {
+                const ListShape keep = k.shape[q];
+                k.shape[q] = apply_step(keep, step);
+                const int kl = keep.lanes > 0 ? keep.lanes : (1 << q);
+                const int cl = k.shape[q].lanes > 0 ? k.shape[q].lanes : (1 << q);
+                if (kl == cl && k.shape[q].map == keep.map && k.shape[q].cap == keep.cap) {
+                    k.shape[q] = keep;
+                    continue;
                 }
-            k.shape[q] = pick;
+                if ((rc = build_tables(b))) break;
+                float ms = 0.f;
+                if ((rc = time_once(ms))) break;
+                if (ms < best) best = ms;
+                else k.shape[q] = keep;
+            }
             if (rc == CIPHER_OK) rc = build_tables(b);
         }
     }
@@ -530,16 +557,19 @@ extern "C" int cipher_batch_autotune(cipher_batch *b, const double *X, double *F
     return rc;
 }
 
-// shape of list q of class k: threads, cap (for reports)
-extern "C" int cipher_batch_shape(const cipher_batch *b, int32_t k, int32_t q, int32_t *ncp, int32_t *lanes,
-                                  int32_t *threads, int32_t *cap) {
+extern "C" int cipher_batch_shape(const cipher_batch *b, int32_t k, int32_t q, int32_t *ncp, int32_t *capacity_lanes,
+                                  cipher_list_shape *shape) {
     if (!b || k < 0 || k >= (int)b->cls.size() || q < 0 || q >= 5)
         return cipher_fail(CIPHER_ERR_ARG, "no such class or list");
     const BatchClass &c = b->cls[k];
     if (ncp) *ncp = c.ncp;
-    if (lanes) *lanes = c.G;
-    if (threads) *threads = c.shape[q].threads;
-    if (cap) *cap = c.shape[q].cap;
+    if (capacity_lanes) *capacity_lanes = c.G;
+    if (shape) {
+        shape->lanes = c.shape[q].lanes > 0 ? c.shape[q].lanes : (1 << q);
+        shape->cap = c.shape[q].cap;
+        shape->map = c.shape[q].map;
+        shape->pad_ = 0;
+    }
     return CIPHER_OK;
 }
 

@@ -3,13 +3,46 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
 #include "kernels.h"
 
-// adaptive explicit RK state: stage vectors, device step size, HIP graph of
-// one attempt (captured for one solution pointer)
+// One coordinate step of the launch-shape search: set one field of a
+// ListShape (0 lanes, 1 block order, 2 grid cap).
+struct ShapeStep {
+    int field, value;
+};
+
+// Steps tried for list q of a mesh with lane capacity G: lanes 2^q..4*2^q
+// (single-phase cells without chemistry keep their one-lane path), both
+// block orders, caps of 4096/16384/65536 blocks.
+static inline std::vector<ShapeStep> shape_steps(int q, int G, int ncp) {
+    std::vector<ShapeStep> out;
+    const int base = 1 << q;
+    if (!(q == 0 && ncp == 1))
+        for (int l = base; l <= std::min(G, base * 4); l *= 2) out.push_back({0, l});
+    for (int m : {0, 1}) out.push_back({1, m});
+    for (int cap : {4096, 16384, 65536}) out.push_back({2, cap});
+    return out;
+}
+
+static inline cipher::ListShape apply_step(cipher::ListShape s, const ShapeStep &st) {
+    if (st.field == 0) s.lanes = st.value;
+    else if (st.field == 1) s.map = st.value;
+    else s.cap = st.value;
+    return s;
+}
+
+// a shape for list q of a mesh with lane capacity G
+static inline bool shape_valid(const cipher_list_shape &s, int q, int G) {
+    const int base = 1 << q;
+    const bool lanes_ok = s.lanes == 0 || ((s.lanes & (s.lanes - 1)) == 0 && s.lanes >= base && s.lanes <= G &&
+                                           s.lanes <= 16);
+    return lanes_ok && s.cap >= 1 && (s.map == 0 || s.map == 1);
+}
+
 struct TsState {
     bool ready = false;
     cipher_ts_params p{};
@@ -41,8 +74,6 @@ struct cipher_ctx {
     int32_t *list = nullptr;        // FP32-mixed recomputation lists (5 * ncells)
     int32_t *bins = nullptr;        // cells by active-set size (5 * ncells)
     int32_t *counts = nullptr;      // list and recomputation-list sizes (RHS_COUNTERS)
-    hipStream_t aux[4] = {nullptr, nullptr, nullptr, nullptr};   // default priority
-    hipEvent_t ev_fork = nullptr, ev_join[4] = {nullptr, nullptr, nullptr, nullptr};
     int32_t *err = nullptr;
     double *partial = nullptr;
     double *norm = nullptr;
@@ -52,6 +83,7 @@ struct cipher_ctx {
     int32_t *h_err = nullptr;
     int npart = 1024;
     TsState ts;                     // adaptive integrator (integrator.hip)
+    cipher::ListShape shape[5];     // launch shapes of the per-size lists
 };
 
 

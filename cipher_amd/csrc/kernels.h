@@ -21,6 +21,17 @@ struct RhsArgs {
     int32_t *redo;            // FP32-mixed: cells recomputed in FP64
     int32_t *redo_count;
     long long ncells;
+    int32_t map;              // block order: 0 as launched, 1 one contiguous range per XCD
+    int32_t pad_;
+};
+
+// Launch shape of one per-size cell list: lanes per cell (at least the
+// list's active-set bound; more lanes take more faces per step), grid cap
+// and block order.  Shapes change timing only, never results.
+struct ListShape {
+    int lanes = 0;            // 0: 2^q for list q
+    int cap = 16384;          // blocks of the list's range
+    int map = 0;              // RhsArgs::map
 };
 
 // per-mesh list counters: 0..4 the per-size lists, 8..12 their FP32-mixed
@@ -37,6 +48,19 @@ static inline size_t rhs_lds_bytes(const DevCfg &c, int G, int cpb) {
     return (size_t)cpb * G * G * (sizeof(int) + (c.has_grad ? 2 * sizeof(double) : 0));
 }
 
+// Block ranges of one merged launch over the per-size lists: range r covers
+// blocks [start[r], start[r+1]) and runs list list[r] with lanes[r] lanes per
+// cell and block order map[r]; with redo set, the ranges run the lists'
+// FP32-mixed recomputation lists instead.
+struct ListRanges {
+    int32_t start[6];
+    int32_t list[5];
+    int32_t lanes[5];
+    int32_t map[5];
+    int32_t n;
+    int32_t redo;
+};
+
 struct RhsLaunch {
     const DevCfg *cfg;
     const double *X;
@@ -48,9 +72,8 @@ struct RhsLaunch {
     int32_t *bins;            // 5 lists of ncells entries (cells by active-set size)
     int32_t *counts;          // RHS_COUNTERS counters
     int32_t *redo;            // 5 recomputation lists of ncells entries
-    hipStream_t aux[4];       // forked streams for the concurrent lists
-    hipEvent_t ev_fork, ev_join[4];
     long long ncells;
+    ListShape shape[5];
 };
 
 cipher_status launch_rhs(const RhsLaunch &L, int precision, hipStream_t st);
@@ -105,10 +128,16 @@ __device__ __forceinline__ int seg_find(const int32_t *__restrict__ seg, int nse
 struct BatchSegs {
     const int32_t *seg = nullptr;     // nseg+1 block offsets
     const int32_t *segmem = nullptr;  // member index of each segment
+    const int32_t *segq = nullptr;    // list of each segment (merged list launches)
     int nseg = 0;
     int nblocks = 0;
     size_t lds = 0;                   // dynamic LDS bytes (max over members)
-    int threads = 256;                // block size
+};
+
+// lanes per cell and block order of the five lists of a class
+struct BatchLists {
+    int32_t lanes[5];
+    int32_t map[5];
 };
 
 // one class of members: same component count and lane capacity G
@@ -116,15 +145,18 @@ struct BatchLaunch {
     const BatchMember *mem;
     const int32_t *live;      // per-member flag (null: all members)
     int ncp, G, nbins;
-    BatchSegs fold, prep, bin, list[5], post, dofs;
+    BatchSegs fold, prep, bin, lists, post, dofs;
+    BatchLists shape;
     const double *X;
     double *F;
     uint32_t *active;
 };
 
-// phase 0: prep records and per-size lists (on st[0]); phase 1: the lists
-// (list b on st[b], in FP32-mixed followed by its FP64 recomputation list)
-cipher_status launch_rhs_batch(const BatchLaunch &L, int phase, int precision, const hipStream_t *st);
+// Every launch of one class's right-hand side on stream st: prep records,
+// per-size lists, all lists of all members in one launch (segments per list
+// and member, largest active sets first), and in FP32-mixed the FP64
+// recomputation lists in one more launch.
+cipher_status launch_rhs_batch(const BatchLaunch &L, int precision, hipStream_t st);
 cipher_status launch_fold_batch(const BatchLaunch &L, hipStream_t st);
 cipher_status launch_post_batch(const BatchLaunch &L, double *X, hipStream_t st);
 cipher_status launch_wrms_batch(const BatchLaunch &L, const double *U, const double *Y, double atol, double rtol,

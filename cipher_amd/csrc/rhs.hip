@@ -20,15 +20,15 @@ cipher_status launch_rhs(const RhsLaunch &L, int precision, hipStream_t st) {
 }
 
 template <int NC>
-cipher_status launch_batch_NC(const BatchLaunch &L, int phase, int precision, const hipStream_t *st);
+cipher_status launch_batch_NC(const BatchLaunch &L, int precision, hipStream_t st);
 
-cipher_status launch_rhs_batch(const BatchLaunch &L, int phase, int precision, const hipStream_t *st) {
+cipher_status launch_rhs_batch(const BatchLaunch &L, int precision, hipStream_t st) {
     switch (L.ncp) {
-    case 1: return launch_batch_NC<1>(L, phase, precision, st);
-    case 2: return launch_batch_NC<2>(L, phase, precision, st);
-    case 3: return launch_batch_NC<3>(L, phase, precision, st);
-    case 4: return launch_batch_NC<4>(L, phase, precision, st);
-    case 5: return launch_batch_NC<5>(L, phase, precision, st);
+    case 1: return launch_batch_NC<1>(L, precision, st);
+    case 2: return launch_batch_NC<2>(L, precision, st);
+    case 3: return launch_batch_NC<3>(L, precision, st);
+    case 4: return launch_batch_NC<4>(L, precision, st);
+    case 5: return launch_batch_NC<5>(L, precision, st);
     default: return CIPHER_ERR_UNSUPPORTED;
     }
 }

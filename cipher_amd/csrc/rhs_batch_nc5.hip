@@ -2,5 +2,5 @@
 #include "rhs_batch.h"
 
 namespace cipher {
-template cipher_status launch_batch_NC<5>(const BatchLaunch &L, int phase, int precision, const hipStream_t *st);
+template cipher_status launch_batch_NC<5>(const BatchLaunch &L, int precision, hipStream_t st);
 }  // namespace cipher

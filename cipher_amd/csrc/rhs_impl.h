@@ -27,7 +27,19 @@
 // rounding band sends its cell to a list that the FP64 kernel recomputes, so
 // the active sets equal the FP64 path's.
 #pragma once
+#include <algorithm>
+
 #include "kernels.h"
+
+// minimum resident 256-thread blocks per CU of the right-hand-side kernels
+// without (MINB1) and with chemistry (MINBN); this bounds their registers
+// per lane: 3 blocks = 3 waves per SIMD = at most 168 VGPRs
+#ifndef CIPHER_RHS_MINB1
+#define CIPHER_RHS_MINB1 3
+#endif
+#ifndef CIPHER_RHS_MINBN
+#define CIPHER_RHS_MINBN 1
+#endif
 
 namespace cipher {
 
@@ -285,7 +297,17 @@ __device__ __forceinline__ void wave_sync() {
 // stages its cell's record in LDS, computes, and writes the cell's F record
 // itself.  In FP32-mixed, cells whose active flags fall inside the FP32
 // rounding band are appended to a.redo for an FP64 pass over the same list.
-// (blocks blk, blk + nblk, ... of the list; a block is 256/G groups)
+// Blocks are dealt to the 8 XCDs round-robin, each XCD with its own L2.
+// Block blk of nblk is given position x*q + min(x, r) + blk/8 of the list
+// order (x = blk mod 8, q = nblk/8, r = nblk mod 8), so each XCD takes one
+// contiguous range of the (Morton-ordered) list and the neighbour records a
+// block reads were mostly fetched by blocks of the same XCD.
+__device__ __forceinline__ long long xcd_block(long long blk, long long nblk) {
+    const long long q = nblk >> 3, r = nblk & 7, x = blk & 7;
+    return x * q + (x < r ? x : r) + (blk >> 3);
+}
+
+// (blocks blk, blk + nblk, ... of the list; a block is blockDim/G groups)
 template <int G, int GMAX, int NC, class R>
 __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, long long blk, long long nblk) {
     constexpr int RM = (GMAX + G - 1) / G;                 // neighbour entries per lane
@@ -301,6 +323,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
     int *pt = ptype + lc * G * G;
     const long long limit = (long long)*a.cells_count;
     const long long stride = nblk * CPB;
+    if (a.map) blk = xcd_block(blk, nblk);
     // the G lanes of a group share the loop trip count; groups of a wave may
     // differ, so nothing below synchronises more than the group's wave
     for (long long j = blk * CPB + lc; j < limit; j += stride) {
@@ -1026,9 +1049,31 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
     }  // cell loop
 }
 
-template <int G, int GMAX, int NC, class R>
-__global__ __launch_bounds__(256, NC == 1 ? 4 : 1) void k_rhs(RhsArgs a, DevCfg c) {
-    rhs_cells<G, GMAX, NC, R>(a, c, blockIdx.x, gridDim.x);
+// All per-size lists of one right-hand side in one launch: the grid is cut
+// into block ranges, one per list (largest active sets first, so the
+// longest-latency cells start earliest), each range striding over its list
+// with that list's lane count.  One launch replaces a fork of concurrent
+// per-list launches and the cross-stream waits it needs.
+template <int GMAX, int NC, class R>
+__global__ __launch_bounds__(256, NC == 1 ? CIPHER_RHS_MINB1 : CIPHER_RHS_MINBN) void k_rhs_all(RhsArgs a, DevCfg c, const int32_t *bins,
+                                                                   int32_t *redo, int32_t *counts, ListRanges rg) {
+    int r = 0;
+    while (r + 1 < rg.n && (int)blockIdx.x >= rg.start[r + 1]) r++;
+    const int q = rg.list[r];
+    const long long N = a.ncells;
+    a.redo = redo + q * N;
+    a.redo_count = counts + 8 + q;
+    a.cells = rg.redo ? a.redo : bins + q * N;
+    a.cells_count = rg.redo ? a.redo_count : counts + q;
+    a.map = rg.map[r];
+    const long long blk = (long long)blockIdx.x - rg.start[r], nblk = rg.start[r + 1] - rg.start[r];
+    switch (rg.lanes[r]) {
+    case 1: rhs_cells<1, GMAX, NC, R>(a, c, blk, nblk); break;
+    case 2: rhs_cells<2, GMAX, NC, R>(a, c, blk, nblk); break;
+    case 4: rhs_cells<4, GMAX, NC, R>(a, c, blk, nblk); break;
+    case 8: if constexpr (GMAX >= 8) rhs_cells<8, GMAX, NC, R>(a, c, blk, nblk); break;
+    default: if constexpr (GMAX >= 16) rhs_cells<16, GMAX, NC, R>(a, c, blk, nblk); break;
+    }
 }
 
 // bin b holds the cells whose active-set size needs 2^b lanes
@@ -1101,60 +1146,33 @@ __global__ void k_fill_i32(int32_t *p, int n, int32_t v) {
 }
 
 
-// One k_rhs launch over a device-counted cell list: blocks of 256 threads in
-// a grid sized for the whole mesh (the list length is only known on the
-// device; blocks past it exit at once), striding when the cap is reached.
-template <int G, int GMAX, int NC, class R>
-static void launch_list(const RhsArgs &a0, const DevCfg &c, const int32_t *cells, const int32_t *count,
-                        long long maxcells, hipStream_t st) {
-    RhsArgs a = a0;
-    a.cells = cells;
-    a.cells_count = count;
-    constexpr int T = 256;
-    constexpr int cpb = T / G;
-    long long nb = (maxcells + cpb - 1) / cpb;
-    if (nb > 16384) nb = 16384;
-    if (nb < 1) nb = 1;
-    hipLaunchKernelGGL((k_rhs<G, GMAX, NC, R>), dim3((unsigned)nb), dim3(T), rhs_lds_bytes(c, G, cpb), st, a, c);
-}
-
-// One list q (cells of 2^q lanes); in FP32-mixed followed, on the same
-// stream, by the FP64 recomputation of the list's cells that the FP32 pass
-// left (counts[8+q] cells at redo + q*N).
-template <int G, int GMAX, int NC, class R>
-static void launch_bin(const RhsArgs &a0, const DevCfg &c, const RhsLaunch &L, int q, hipStream_t st) {
-    const long long N = L.ncells;
-    RhsArgs a = a0;
-    a.redo = L.redo + q * N;
-    a.redo_count = L.counts + 8 + q;
-    launch_list<G, GMAX, NC, R>(a, c, L.bins + q * N, L.counts + q, N, st);
-    if constexpr (sizeof(R) == 4) launch_list<G, GMAX, NC, double>(a, c, a.redo, a.redo_count, N, st);
-}
-
-// The lists run concurrently: the heaviest (largest active sets, longest
-// per-cell latency) on the caller's stream, the others on forked streams, so
-// one list's tail overlaps the others' work.
-template <int GMAX, int NC, class R>
-static bool launch_bins(const RhsArgs &a, const DevCfg &c, const RhsLaunch &L, hipStream_t st) {
-    bool ok = hipEventRecord(L.ev_fork, st) == hipSuccess;
-    const hipStream_t *s = L.aux;
-    for (int i = 0; i < 4; i++) ok = ok && hipStreamWaitEvent(s[i], L.ev_fork, 0) == hipSuccess;
-    if (!ok) return false;
-    if constexpr (GMAX >= 16) {
-        launch_bin<16, GMAX, NC, R>(a, c, L, 4, st);
-        launch_bin<8, GMAX, NC, R>(a, c, L, 3, s[3]);
-    } else if constexpr (GMAX >= 8) {
-        launch_bin<8, GMAX, NC, R>(a, c, L, 3, st);
+// Ranges of the merged launch for lists 0..nbins-1 (launch order: largest
+// active sets first); *lds: the largest LDS need over the ranges.
+static inline ListRanges make_ranges(const ListShape *shape, long long N, const DevCfg &c, int nbins, size_t *lds,
+                                     long long *nblocks) {
+    ListRanges rg{};
+    rg.n = nbins;
+    long long tot = 0;
+    size_t l = 0;
+    for (int r = 0; r < nbins; r++) {
+        const int q = nbins - 1 - r;
+        const ListShape &sh = shape[q];
+        const int lanes = sh.lanes > 0 ? sh.lanes : (1 << q);
+        const int cpb = 256 / lanes;
+        long long nb = (N + cpb - 1) / cpb;
+        if (nb > sh.cap) nb = sh.cap;
+        if (nb < 1) nb = 1;
+        rg.start[r] = (int32_t)tot;
+        rg.list[r] = q;
+        rg.lanes[r] = lanes;
+        rg.map[r] = sh.map;
+        tot += nb;
+        l = std::max(l, rhs_lds_bytes(c, lanes, cpb));
     }
-    launch_bin<4, GMAX, NC, R>(a, c, L, 2, GMAX >= 8 ? s[2] : st);
-    launch_bin<2, GMAX, NC, R>(a, c, L, 1, s[1]);
-    launch_bin<1, GMAX, NC, R>(a, c, L, 0, s[0]);
-    ok = hipGetLastError() == hipSuccess;
-    for (int i = 0; i < 4; i++) {
-        ok = ok && hipEventRecord(L.ev_join[i], s[i]) == hipSuccess;
-        ok = ok && hipStreamWaitEvent(st, L.ev_join[i], 0) == hipSuccess;
-    }
-    return ok;
+    rg.start[nbins] = (int32_t)tot;
+    *lds = l;
+    *nblocks = tot;
+    return rg;
 }
 
 // ---------------------------------------------------------------------------
@@ -1175,16 +1193,25 @@ static cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st)
     a.cell_off = L.cell_off; a.adj = L.adj; a.bcell_off = L.bcell_off; a.bface_id = L.bface_id;
     a.cm = (const double *)L.cm; a.ncells = N;
     a.cells = nullptr; a.cells_count = nullptr;
-    a.redo = nullptr; a.redo_count = nullptr;
+    a.redo = nullptr; a.redo_count = nullptr; a.map = 0; a.pad_ = 0;
     // counts[0..4]: list sizes, counts[8..12]: their FP32-mixed recomputation lists
     hipLaunchKernelGGL(k_fill_i32<0>, dim3(1), dim3(64), 0, st, L.counts, RHS_COUNTERS, 0);
     const int nbins = G >= 16 ? 5 : (G >= 8 ? 4 : 3);
     hipLaunchKernelGGL(k_bin<0>, dim3((unsigned)((N + BIN_TILE - 1) / BIN_TILE)), dim3(256), 0, st, L.X, N, c, nbins,
                        L.bins, N, L.counts);
+    size_t lds = 0;
+    long long tot = 0;
+    ListRanges rg = make_ranges(L.shape, N, c, nbins, &lds, &tot);
     if (precision == CIPHER_FP32_MIXED) {
-        if (!launch_bins<G, NC, float>(a, c, L, st)) return CIPHER_ERR_HIP;
+        hipLaunchKernelGGL((k_rhs_all<G, NC, float>), dim3((unsigned)tot), dim3(256), lds, st, a, c, L.bins, L.redo,
+                           L.counts, rg);
+        // FP64 recomputation of the cells whose flags sat in the FP32 band
+        rg.redo = 1;
+        hipLaunchKernelGGL((k_rhs_all<G, NC, double>), dim3((unsigned)tot), dim3(256), lds, st, a, c, L.bins, L.redo,
+                           L.counts, rg);
     } else {
-        if (!launch_bins<G, NC, double>(a, c, L, st)) return CIPHER_ERR_HIP;
+        hipLaunchKernelGGL((k_rhs_all<G, NC, double>), dim3((unsigned)tot), dim3(256), lds, st, a, c, L.bins, L.redo,
+                           L.counts, rg);
     }
     return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
 }

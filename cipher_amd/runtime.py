@@ -105,6 +105,30 @@ class Context:
         abi.check(self.lib.cipher_axpy_n(self.h, X.numel(), _ptr(Y), _ptr(X), nk, a, ks), "cipher_axpy_n")
         return Y
 
+    # ---- launch shapes ------------------------------------------------------
+    def shapes(self):
+        """(lanes, cap, map) of the five per-size lists."""
+        out = []
+        sh = abi.ListShape()
+        for q in range(5):
+            abi.check(self.lib.cipher_get_shape(self.h, q, C.byref(sh)), "cipher_get_shape")
+            out.append(sh.tuple())
+        return out
+
+    def set_shape(self, q, lanes, cap=16384, map=0):
+        sh = abi.ListShape(lanes, cap, map, 0)
+        abi.check(self.lib.cipher_set_shape(self.h, q, C.byref(sh)), "cipher_set_shape")
+
+    def autotune(self, X, reps=5):
+        """Pick the list shapes by timing on X (current precision); returns
+        (ms per RHS, shapes)."""
+        self._check_state(X, "X")
+        F = self.torch.empty_like(X)
+        self._stream()
+        ms = C.c_double()
+        abi.check(self.lib.cipher_autotune(self.h, _ptr(X), _ptr(F), reps, C.byref(ms)), "cipher_autotune")
+        return ms.value, self.shapes()
+
     # ---- adaptive RK (TSRK 3bs + step controller, src/cipher.c:1097-1115) ----
     def ts_setup(self, params):
         self.ts_params = params
@@ -227,7 +251,7 @@ class Batch:
 
     def autotune(self, X, reps=5):
         """Pick launch shapes by timing on X; returns (ms per batched RHS,
-        [(components, lanes, list, threads, cap), ...])."""
+        [(components, lane capacity, list, lanes, cap, map), ...])."""
         self._check(X, "X")
         F = self.torch.empty_like(X)
         self._stream()
@@ -237,17 +261,16 @@ class Batch:
         return ms.value, self.shapes()
 
     def shapes(self):
+        """[(components, lane capacity, list, lanes, cap, map), ...]"""
         out = []
         k = 0
-        while True:
-            v = [C.c_int32() for _ in range(4)]
-            if self.lib.cipher_batch_shape(self.h, k, 0, *[C.byref(x) for x in v]):
-                break
-            ncp, lanes = v[0].value, v[1].value
-            nl = 5 if lanes >= 16 else (4 if lanes >= 8 else 3)
+        ncp, lanes, sh = C.c_int32(), C.c_int32(), abi.ListShape()
+        while self.lib.cipher_batch_shape(self.h, k, 0, C.byref(ncp), C.byref(lanes), C.byref(sh)) == 0:
+            nl = 5 if lanes.value >= 16 else (4 if lanes.value >= 8 else 3)
             for q in range(nl):
-                abi.check(self.lib.cipher_batch_shape(self.h, k, q, *[C.byref(x) for x in v]), "cipher_batch_shape")
-                out.append((ncp, lanes, q, v[2].value, v[3].value))
+                abi.check(self.lib.cipher_batch_shape(self.h, k, q, C.byref(ncp), C.byref(lanes), C.byref(sh)),
+                          "cipher_batch_shape")
+                out.append((ncp.value, lanes.value, q) + sh.tuple())
             k += 1
         return out
 

@@ -317,6 +317,27 @@ int cipher_ts_accept(cipher_ctx *ctx, double *X_dev);
 int cipher_ts_solve(cipher_ctx *ctx, double *X_dev, double t0, double t_end, double *dt_inout,
                     int64_t max_steps, cipher_ts_stats *stats);
 
+/* ---- launch shapes ------------------------------------------------------- */
+/* Shape of one of the five per-size cell lists (list q holds the cells with at
+ * most 2^q active phases; all lists run in one launch of 256-thread blocks,
+ * one block range per list): lanes per cell (0 = 2^q; else a power of two
+ * between 2^q and the capacity's lane count, at most 16), the list's block
+ * cap (its range strides beyond it), and the block order (0 as launched, 1
+ * one contiguous part of the list per XCD).  Shapes change timing only,
+ * never results.  No counterpart in the reference. */
+typedef struct cipher_list_shape {
+    int32_t lanes, cap, map, pad_;
+} cipher_list_shape;
+
+int cipher_get_shape(const cipher_ctx *ctx, int32_t q, cipher_list_shape *out);
+int cipher_set_shape(cipher_ctx *ctx, int32_t q, const cipher_list_shape *shape);
+
+/* Pick every list's shape by timing candidates on the state X_dev (F_dev is
+ * scratch of the same size): coordinate descent, each candidate timed as the
+ * median of reps right-hand sides.  *ms_out (may be NULL): the best time of
+ * one right-hand side in the current precision. */
+int cipher_autotune(cipher_ctx *ctx, const double *X_dev, double *F_dev, int32_t reps, double *ms_out);
+
 /* ---- batched ensembles ---------------------------------------------------
  * No counterpart in the reference, which integrates one microstructure per
  * process.  A batch runs many independent contexts (each with its own
@@ -359,9 +380,10 @@ int cipher_batch_wrms_norm(cipher_batch *b, const double *U_dev, const double *Y
  * best time of one batched right-hand side. */
 int cipher_batch_autotune(cipher_batch *b, const double *X_dev, double *F_dev, int32_t reps,
                           double *ms_out);
-/* Report class k's component count, lanes per cell and list q's shape. */
-int cipher_batch_shape(const cipher_batch *b, int32_t k, int32_t q, int32_t *ncp, int32_t *lanes,
-                       int32_t *threads, int32_t *cap);
+/* Report class k's component count and lane capacity and the shape of its
+ * list q. */
+int cipher_batch_shape(const cipher_batch *b, int32_t k, int32_t q, int32_t *ncp, int32_t *capacity_lanes,
+                       cipher_list_shape *shape);
 
 /* Per-member adaptive integration: cipher_ts_solve for every member at once,
  * each with its own step size, controller and end time; an attempt advances

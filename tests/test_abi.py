@@ -46,14 +46,15 @@ def test_library_exports_every_declared_symbol(lib):
 def test_struct_layouts_match_the_c_compiler(tmp_path):
     names = ["cipher_tseries", "cipher_rk", "cipher_mobility", "cipher_source", "cipher_material",
              "cipher_interface", "cipher_bc", "cipher_config", "cipher_layout", "cipher_mesh",
-             "cipher_ts_params", "cipher_ts_ctrl", "cipher_ts_stats"]
+             "cipher_ts_params", "cipher_ts_ctrl", "cipher_ts_stats", "cipher_list_shape"]
     mirrors = [abi.TSeries, abi.RK, abi.Mobility, abi.Source, abi.Material, abi.Interface, abi.BC,
-               abi.Config, abi.Layout, abi.Mesh, abi.TsParams, abi.TsCtrl, abi.TsStats]
+               abi.Config, abi.Layout, abi.Mesh, abi.TsParams, abi.TsCtrl, abi.TsStats, abi.ListShape]
     fields = [("cipher_config", "junctionpenalty", abi.Config), ("cipher_config", "gradient_calculation", abi.Config),
               ("cipher_material", "model", abi.Material), ("cipher_material", "source", abi.Material),
               ("cipher_interface", "e_act", abi.Interface), ("cipher_bc", "boundary_id", abi.BC),
               ("cipher_mesh", "ncells", abi.Mesh), ("cipher_tseries", "exp", abi.TSeries),
-              ("cipher_ts_params", "use_graph", abi.TsParams), ("cipher_ts_stats", "t", abi.TsStats)]
+              ("cipher_ts_params", "use_graph", abi.TsParams), ("cipher_ts_stats", "t", abi.TsStats),
+              ("cipher_list_shape", "map", abi.ListShape)]
     src = tmp_path / "sz.c"
     body = "".join(f'printf("%zu\\n", sizeof({n}));' for n in names)
     body += "".join(f'printf("%zu\\n", offsetof({s}, {f}));' for s, f, _ in fields)

@@ -189,7 +189,7 @@ def test_batch_autotune_keeps_results():
     ms, shapes = b.autotune(X, reps=3)
     print(f"[batch-tune] {ms:.4f} ms, shapes {shapes}", flush=True)
     assert ms > 0.0 and shapes
-    for ncp, lanes, q, threads, cap in shapes:
-        assert threads in (64, 128, 256) and threads >= (1 << q)
+    for ncp, capl, q, lanes, cap, mp in shapes:
+        assert (1 << q) <= lanes <= capl and mp in (0, 1) and cap >= 1
     assert np.array_equal(_host(b.rhs(X)), F0)
     b.close()
