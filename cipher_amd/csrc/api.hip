@@ -142,9 +142,11 @@ extern "C" int cipher_ctx_create(const cipher_config *cfg, int precision, cipher
     if (precision != CIPHER_FP64 && precision != CIPHER_FP32_MIXED) return fail(CIPHER_ERR_ARG, "precision");
     int rc = validate(cfg);
     if (rc) return rc;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
     cipher_ctx *c = new cipher_ctx();
     c->precision = precision;
-    HIPCHK(hipGetDevice(&c->device));
+    c->device = dev;
     cipher_layout_of(cfg, &c->L);
     if (c->L.pf_size > 16) {
         delete c;
@@ -401,7 +403,7 @@ extern "C" int cipher_upload_mesh(cipher_ctx *c, const cipher_mesh *m) {
     return CIPHER_OK;
 }
 
-int cipher_rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active) {
+int cipher_rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active, int bins_mode) {
     if (!c || !X || !F) return fail(CIPHER_ERR_ARG, "null argument");
     if (!c->cell_off) return fail(CIPHER_ERR_STATE, "no mesh uploaded");
     RhsLaunch L;
@@ -420,6 +422,7 @@ int cipher_rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active)
     L.counts = c->counts;
     L.ncells = c->ncells;
     for (int q = 0; q < 5; q++) L.shape[q] = c->shape[q];
+    L.bins_mode = bins_mode;
     if (c->fold) {
         cipher_status f = launch_tsfold(X, c->ncells, c->dc, c->tsv, c->stream);
         if (f != CIPHER_OK) return fail(f, "fold launch");

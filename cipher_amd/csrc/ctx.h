@@ -89,7 +89,8 @@ struct cipher_ctx {
 
 // shared between the translation units of the C ABI
 int cipher_fail(int code, const std::string &msg);
-int cipher_rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active);
+// bins_mode: see RhsLaunch (0 for a right-hand side on its own)
+int cipher_rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active, int bins_mode = 0);
 int cipher_post_common(cipher_ctx *c, double *X, const int32_t *phase, bool setup);
 void cipher_ts_free(cipher_ctx *c);
 // a captured attempt bakes in the precision, stream and temperature rate

@@ -143,13 +143,15 @@ static int enqueue_attempt(cipher_ctx *c, const double *X) {
     const double *k2[1] = {s.K[1]};
     const double *k123[3] = {s.K[0], s.K[1], s.K[2]};
     const double *k1234[4] = {s.K[0], s.K[1], s.K[2], s.K[3]};
-    if ((rc = cipher_rhs_impl(c, X, s.K[0], nullptr))) return rc;
+    // the stage values keep X's active sets (the rate of an AS entry is 0),
+    // so the cells are binned once per attempt and the lists reused
+    if ((rc = cipher_rhs_impl(c, X, s.K[0], nullptr, 1))) return rc;
     if (launch_axpy_dt(n, s.Y, X, s.dt_dev, 1, A2, k1, st)) return cipher_fail(CIPHER_ERR_HIP, "stage 2");
-    if ((rc = cipher_rhs_impl(c, s.Y, s.K[1], nullptr))) return rc;
+    if ((rc = cipher_rhs_impl(c, s.Y, s.K[1], nullptr, 2))) return rc;
     if (launch_axpy_dt(n, s.Y, X, s.dt_dev, 1, A3, k2, st)) return cipher_fail(CIPHER_ERR_HIP, "stage 3");
-    if ((rc = cipher_rhs_impl(c, s.Y, s.K[2], nullptr))) return rc;
+    if ((rc = cipher_rhs_impl(c, s.Y, s.K[2], nullptr, 2))) return rc;
     if (launch_axpy_dt(n, s.Xn, X, s.dt_dev, 3, B, k123, st)) return cipher_fail(CIPHER_ERR_HIP, "stage 4");
-    if ((rc = cipher_rhs_impl(c, s.Xn, s.K[3], nullptr))) return rc;
+    if ((rc = cipher_rhs_impl(c, s.Xn, s.K[3], nullptr, 3))) return rc;
     if (launch_axpy_dt(n, s.Xh, X, s.dt_dev, 4, BH, k1234, st)) return cipher_fail(CIPHER_ERR_HIP, "embedded");
     if (launch_wrms(s.Xn, s.Xh, n, s.p.atol, s.p.rtol, c->partial, wrms_parts(n), c->norm, st))
         return cipher_fail(CIPHER_ERR_HIP, "norm");

@@ -74,6 +74,9 @@ struct RhsLaunch {
     int32_t *redo;            // 5 recomputation lists of ncells entries
     long long ncells;
     ListShape shape[5];
+    // 0 or 1: bin the cells; 2 or 3: reuse the lists of the previous call,
+    // whose state had the same active sets (stages of one attempt)
+    int bins_mode;
 };
 
 cipher_status launch_rhs(const RhsLaunch &L, int precision, hipStream_t st);

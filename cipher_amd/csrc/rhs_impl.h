@@ -321,25 +321,50 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
     const int lc = threadIdx.x / G;
     const int slot = threadIdx.x & (G - 1);
     int *pt = ptype + lc * G * G;
-    const long long limit = (long long)*a.cells_count;
+    const long long limit = min((long long)*a.cells_count, a.ncells);
     const long long stride = nblk * CPB;
     if (a.map) blk = xcd_block(blk, nblk);
+    if constexpr (G == 1 && NC == 1) {
+        if (!c.has_tcond && !c.has_theat) {
+            // Single-phase cells without chemistry, conduction or heat terms:
+            // zero phase rate and Laplacian, so F is 0 except the imposed
+            // temperature rate (src/cipher.c:558-568).  A wave writes its 64
+            // cells' records together: one contiguous range when the cells
+            // are consecutive, else one record per step (D lanes, two cache
+            // lines) rather than one double per lane per step (64 lines).
+            const int lane = (int)(threadIdx.x & 63);
+            const double tr = c.has_trate ? 0.0 + c.trate : 0.0;
+            for (long long j0 = blk * CPB + (lc & ~63); j0 < limit; j0 += stride) {
+                const long long j = j0 + lane;
+                const int cell = j < limit ? a.cells[j] : -1;
+                if (a.active && cell >= 0) a.active[cell] = 0u;
+                const int c0 = __shfl(cell, 0, 64);
+                const int cnt = (int)min((long long)64, limit - j0);
+                const bool contig = __all(cell < 0 || cell == c0 + lane);
+                if (contig) {
+                    double *F0 = a.F + (long long)c0 * D;
+                    const int tot = cnt * D;
+                    const int step = 64 % D;
+                    int rem = lane % D;
+                    for (int k = lane; k < tot; k += 64) {
+                        F0[k] = (c.has_trate && rem == c.tmo) ? tr : 0.0;
+                        rem += step;
+                        if (rem >= D) rem -= D;
+                    }
+                } else {
+                    for (int i = 0; i < cnt; i++) {
+                        const int ci = __shfl(cell, i, 64);
+                        if (lane < D) a.F[(long long)ci * D + lane] = (c.has_trate && lane == c.tmo) ? tr : 0.0;
+                    }
+                }
+            }
+            return;
+        }
+    }
     // the G lanes of a group share the loop trip count; groups of a wave may
     // differ, so nothing below synchronises more than the group's wave
     for (long long j = blk * CPB + lc; j < limit; j += stride) {
         const long long cell = (long long)a.cells[j];
-        if constexpr (G == 1 && NC == 1) {
-            // single-phase cell without chemistry, conduction or heat terms:
-            // zero phase rate and Laplacian, so F is 0 except the imposed
-            // temperature rate (src/cipher.c:558-568)
-            if (!c.has_tcond && !c.has_theat) {
-                double *Fc = a.F + cell * D;
-                for (int k = 0; k < D; k++) Fc[k] = 0.0;
-                if (c.has_trate) Fc[c.tmo] = 0.0 + c.trate;
-                if (a.active) a.active[cell] = 0u;
-                continue;
-            }
-        }
         // the group's lanes read the cell's record together (same or
         // consecutive addresses), so it is served by one or two cache lines
         const double *x = a.X + cell * D;
@@ -1194,11 +1219,18 @@ static cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st)
     a.cm = (const double *)L.cm; a.ncells = N;
     a.cells = nullptr; a.cells_count = nullptr;
     a.redo = nullptr; a.redo_count = nullptr; a.map = 0; a.pad_ = 0;
-    // counts[0..4]: list sizes, counts[8..12]: their FP32-mixed recomputation lists
-    hipLaunchKernelGGL(k_fill_i32<0>, dim3(1), dim3(64), 0, st, L.counts, RHS_COUNTERS, 0);
+    // counts[0..4]: list sizes, counts[8..12]: their FP32-mixed recomputation
+    // lists.  A call that bins clears them all; a call that reuses the lists
+    // of the previous one (same active sets) clears only the recomputation
+    // counters, and only in FP32-mixed.
     const int nbins = G >= 16 ? 5 : (G >= 8 ? 4 : 3);
-    hipLaunchKernelGGL(k_bin<0>, dim3((unsigned)((N + BIN_TILE - 1) / BIN_TILE)), dim3(256), 0, st, L.X, N, c, nbins,
-                       L.bins, N, L.counts);
+    if (L.bins_mode < 2) {
+        hipLaunchKernelGGL(k_fill_i32<0>, dim3(1), dim3(64), 0, st, L.counts, RHS_COUNTERS, 0);
+        hipLaunchKernelGGL(k_bin<0>, dim3((unsigned)((N + BIN_TILE - 1) / BIN_TILE)), dim3(256), 0, st, L.X, N, c,
+                           nbins, L.bins, N, L.counts);
+    } else if (precision == CIPHER_FP32_MIXED) {
+        hipLaunchKernelGGL(k_fill_i32<0>, dim3(1), dim3(64), 0, st, L.counts + 8, RHS_COUNTERS - 8, 0);
+    }
     size_t lds = 0;
     long long tot = 0;
     ListRanges rg = make_ranges(L.shape, N, c, nbins, &lds, &tot);
