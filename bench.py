@@ -125,6 +125,52 @@ def prepare_state(ctx, wl, steps):
     return X, evolved, nact
 
 
+def trajectory_fp32_vs_fp64(ctx, wl, X0, steps):
+    """The FP32-mixed adaptive trajectory against the FP64 one from the same
+    state: FP64 runs `steps` accepted steps to a time t1, FP32-mixed runs to
+    the same t1 (exact final time); reported: accepted/rejected step counts
+    and the max and RMS differences of phase fraction (where both runs hold
+    the phase), composition (output fields) and chemical potential."""
+    import torch
+    from cipher_amd import abi
+    L = wl.layout
+    D = L["ndof"]
+    nc = wl.cfg.ncp
+    params = abi.ts_params(1e-4, 1e-4, 1e-12, 1e-3, use_graph=True)
+    res = {}
+    for prec in (abi.FP64, abi.FP32_MIXED):
+        ctx.set_precision(prec)
+        ctx.ts_setup(params)
+        X = X0.clone()
+        if prec == abi.FP64:
+            st, _ = ctx.ts_solve(X, 0.0, 1e-3, 1e-12, max_steps=steps)
+            t1 = st.t
+        else:
+            st, _ = ctx.ts_solve(X, 0.0, t1, 1e-12)
+        comp = ctx.output(X, [(abi.OUT_COMP, c) for c in range(nc)]).cpu().numpy()
+        res[prec] = (st, X.cpu().numpy().reshape(-1, D), comp)
+    ctx.set_precision(abi.FP64)
+    s64, X64, c64 = res[abi.FP64]
+    s32, X32, c32 = res[abi.FP32_MIXED]
+    pf = slice(L["pf_offset"], L["pf_offset"] + L["pf_size"])
+    n64 = np.rint(X64[:, 0]).astype(int)
+    same = np.all(X64[:, :L["as_size"]] == X32[:, :L["as_size"]], axis=1)
+    slot = np.arange(L["pf_size"])[None, :] < n64[:, None]
+    dphi = np.abs(X32[:, pf] - X64[:, pf])[same[:, None] & slot]
+    dc = np.abs(c32 - c64).ravel()
+    dmu = np.abs(X32[:, L["dp_offset"]:L["dp_offset"] + L["dp_size"]] - X64[:, L["dp_offset"]:L["dp_offset"] + L["dp_size"]])
+    mus = np.abs(X64[:, L["dp_offset"]:L["dp_offset"] + L["dp_size"]]).max()
+
+    def mr(a):
+        return {"max": float(a.max()) if a.size else 0.0, "rms": float(np.sqrt(np.mean(a * a))) if a.size else 0.0}
+
+    return {"t_end": t1, "fp64_steps": [s64.steps, s64.rejected], "fp32_mixed_steps": [s32.steps, s32.rejected],
+            "accepted_step_difference": int(s32.steps - s64.steps),
+            "cells_with_equal_active_sets": float(same.mean()),
+            "phase_fraction_abs": mr(dphi), "composition_abs": mr(dc),
+            "chemical_potential_rel": {k: v / mus for k, v in mr(dmu.ravel()).items()}}
+
+
 def rel_errors(A, B, L):
     """Normwise relative error of A against B per field (max|A-B| / max|B|)."""
     pf = slice(L["pf_offset"], L["pf_offset"] + L["pf_size"])
@@ -302,6 +348,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of autotuned ones")
+    ap.add_argument("--traj-steps", type=int, default=20,
+                    help="accepted steps of the FP32-mixed vs FP64 trajectory comparison (workloads with chemistry)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -370,6 +418,7 @@ def main():
 
     # FP32-mixed error against FP64, per field (normwise relative)
     rel = rel_errors(F32.view(-1, D).cpu().numpy(), F64.view(-1, D).cpu().numpy(), L)
+    traj = trajectory_fp32_vs_fp64(ctx, wl, X, args.traj_steps) if L["dp_size"] and args.traj_steps > 0 else None
 
     bytes_alg = algorithmic_bytes(N, NF, D)
     rate64 = N / (ms64 * 1e-3)
@@ -412,6 +461,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_alg},
         "fp32_mixed": {"value": rate32, "ms_per_step": ms32, "achieved_gbs": gbs32,
                        "frac": gbs32 / HBM_PEAK_GBS, "speedup_vs_fp64": ms64 / ms32, "rel_err_vs_fp64": rel},
+        "trajectory_fp32_vs_fp64": traj,
         "list_shapes": {"autotuned": not args.no_tune,
                         "fp64": shapes.get(abi.FP64), "fp32_mixed": shapes.get(abi.FP32_MIXED)},
         "cpu_baseline": cpu,

@@ -119,6 +119,10 @@ class TsStats(C.Structure):
                 ("dt", C.c_double), ("err", C.c_double)]
 
 
+OUT_PHASEID, OUT_MATID, OUT_INTERFACEID, OUT_CHEMPOT, OUT_COMP, OUT_PHI, OUT_TEMPERATURE = range(1, 8)
+MAX_OUTPUTS = 32
+
+
 class ListShape(C.Structure):
     _fields_ = [("lanes", C.c_int32), ("cap", C.c_int32), ("map", C.c_int32), ("pad_", C.c_int32)]
 
@@ -174,6 +178,7 @@ def _declare(lib):
         "cipher_get_shape": (i32, [vp, i32, C.POINTER(ListShape)]),
         "cipher_set_shape": (i32, [vp, i32, C.POINTER(ListShape)]),
         "cipher_autotune": (i32, [vp, vp, vp, i32, C.POINTER(C.c_double)]),
+        "cipher_output": (i32, [vp, vp, i32, vp, vp]),
         "cipher_batch_ts_setup": (i32, [vp, C.POINTER(TsParams)]),
         "cipher_batch_ts_solve": (i32, [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                         C.POINTER(C.c_double), i64, C.POINTER(TsStats)]),
@@ -193,7 +198,7 @@ EXPORTED = ("cipher_layout_of", "cipher_ctx_create", "cipher_ctx_destroy", "ciph
             "cipher_batch_create", "cipher_batch_destroy", "cipher_batch_offsets", "cipher_batch_set_precision",
             "cipher_batch_set_stream", "cipher_batch_rhs", "cipher_batch_rhs_active", "cipher_batch_poststep",
             "cipher_batch_wrms_norm", "cipher_batch_autotune", "cipher_batch_shape", "cipher_batch_ts_setup",
-            "cipher_batch_ts_solve", "cipher_get_shape", "cipher_set_shape", "cipher_autotune")
+            "cipher_batch_ts_solve", "cipher_get_shape", "cipher_set_shape", "cipher_autotune", "cipher_output")
 
 
 def load_library(path=None):

@@ -273,6 +273,7 @@ def load_config(text, loadcase=0):
     cfg.n_sites = _atoi(nsites_hdr[0]) if nsites_hdr else 0
     cfg.dim, cfg.grid, cfg.size, cfg.npf, cfg.ncp = dim, grid, size, npf, ncp
     cfg.component_names, cfg.material_names, cfg.interface_names = comps, matnames, ifnames
+    cfg.output_names = [str(v) for v in doc.prop("header", "outputs")]
     ndp = ncp - 1
 
     mats = []
@@ -568,3 +569,36 @@ def load_config(text, loadcase=0):
     s.gradient_calculation = cfg.gradient_calculation
     cfg.pools = pools
     return cfg
+
+
+def output_fields(cfg, names=None):
+    """Output names of the YAML header (src/init.c:258-265) as
+    (kind, index) pairs, matched the way the output step matches them
+    (src/cipher.c:1006-1076): exact "phaseid", "matid", "interfaceid",
+    "temperature"; "<component>_chempot" before "<component>_c" (the first
+    '_'-separated token names the component; no match gives index -1, an
+    output left at 0); "<id>_phi" takes atoi of the first token."""
+    names = cfg.output_names if names is None else names
+    comps = cfg.component_names
+    out = []
+    for name in names:
+        tok = name.split("_")[0]
+        if name == "phaseid":
+            out.append((abi.OUT_PHASEID, 0))
+        elif name == "matid":
+            out.append((abi.OUT_MATID, 0))
+        elif name == "interfaceid":
+            out.append((abi.OUT_INTERFACEID, 0))
+        elif "_chempot" in name:
+            idx = [c for c in range(cfg.ncp - 1) if comps[c] == tok]
+            out.append((abi.OUT_CHEMPOT, idx[-1] if idx else -1))
+        elif "_c" in name:
+            idx = [c for c in range(cfg.ncp) if comps[c] == tok]
+            out.append((abi.OUT_COMP, idx[-1] if idx else -1))
+        elif "_phi" in name:
+            out.append((abi.OUT_PHI, _atoi(tok)))
+        elif name == "temperature":
+            out.append((abi.OUT_TEMPERATURE, 0))
+        else:
+            raise ValueError(f"unknown output field {name!r}")
+    return out

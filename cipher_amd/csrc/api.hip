@@ -443,6 +443,30 @@ extern "C" int cipher_rhs_active(cipher_ctx *c, double t, const double *X, doubl
     return cipher_rhs_impl(c, X, F, active);
 }
 
+extern "C" int cipher_output(cipher_ctx *c, const double *X, int32_t nfields, const cipher_output_field *fields,
+                             double *out) {
+    if (!c || !X || !out || (nfields > 0 && !fields)) return fail(CIPHER_ERR_ARG, "null argument");
+    if (nfields < 0 || nfields > CIPHER_MAX_OUTPUTS) return fail(CIPHER_ERR_ARG, "number of output fields");
+    if (!c->cell_off) return fail(CIPHER_ERR_STATE, "no mesh uploaded");
+    OutFields f{};
+    f.n = nfields;
+    for (int k = 0; k < nfields; k++) {
+        if (fields[k].kind < CIPHER_OUT_PHASEID || fields[k].kind > CIPHER_OUT_TEMPERATURE)
+            return fail(CIPHER_ERR_ARG, "unknown output field kind");
+        f.kind[k] = fields[k].kind;
+        f.index[k] = fields[k].index;
+        if (fields[k].kind == CIPHER_OUT_COMP) f.need_comp = 1;
+    }
+    if (nfields == 0) return CIPHER_OK;
+    if (c->fold && f.need_comp) {
+        cipher_status s = launch_tsfold(X, c->ncells, c->dc, c->tsv, c->stream);
+        if (s != CIPHER_OK) return fail(s, "fold launch");
+    }
+    cipher_status s = launch_output(c->dc, X, (double *)c->cm, c->ncells, f, out, c->stream);
+    if (s != CIPHER_OK) return fail(s, std::string("output launch: ") + hipGetErrorString(hipGetLastError()));
+    return CIPHER_OK;
+}
+
 static int lanes_cap(const cipher_ctx *c) {
     const int P = c->L.pf_size;
     return P <= 4 ? 4 : (P <= 8 ? 8 : 16);

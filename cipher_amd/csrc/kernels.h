@@ -80,6 +80,16 @@ struct RhsLaunch {
 };
 
 cipher_status launch_rhs(const RhsLaunch &L, int precision, hipStream_t st);
+
+// derived output fields (kind/index pairs of cipher_output_field)
+struct OutFields {
+    int32_t kind[CIPHER_MAX_OUTPUTS];
+    int32_t index[CIPHER_MAX_OUTPUTS];
+    int32_t n;
+    int32_t need_comp;        // some field is a composition (the prep records are needed)
+};
+cipher_status launch_output(const DevCfg &c, const double *X, double *cm, long long ncells, const OutFields &f,
+                            double *out, hipStream_t st);
 size_t prep_record_bytes(int ncp);
 cipher_status launch_tsfold(const double *X, long long ncells, const DevCfg &c, double *tsv, hipStream_t st);
 

@@ -33,6 +33,22 @@ cipher_status launch_rhs_batch(const BatchLaunch &L, int precision, hipStream_t 
     }
 }
 
+template <int NC>
+cipher_status launch_output_NC(const DevCfg &c, const double *X, double *cm, long long N, const OutFields &f,
+                               double *out, hipStream_t st);
+
+cipher_status launch_output(const DevCfg &c, const double *X, double *cm, long long ncells, const OutFields &f,
+                            double *out, hipStream_t st) {
+    switch (c.ncp) {
+    case 1: return launch_output_NC<1>(c, X, cm, ncells, f, out, st);
+    case 2: return launch_output_NC<2>(c, X, cm, ncells, f, out, st);
+    case 3: return launch_output_NC<3>(c, X, cm, ncells, f, out, st);
+    case 4: return launch_output_NC<4>(c, X, cm, ncells, f, out, st);
+    case 5: return launch_output_NC<5>(c, X, cm, ncells, f, out, st);
+    default: return CIPHER_ERR_UNSUPPORTED;
+    }
+}
+
 // fold every T-series at the temperature of the first cell (deterministic:
 // the same code as the direct evaluation)
 __global__ void k_tsfold(const double *X, long long ncells, DevCfg c, double *tsv) {

@@ -265,6 +265,95 @@ __global__ __launch_bounds__(256) void k_prep(const double *__restrict__ X, doub
 }
 
 // ---------------------------------------------------------------------------
+// k_output: the fields CIPHER writes at an output step (src/cipher.c:1005-1080)
+// for one cell per group of G lanes: the largest interpolant in slot order
+// (strict '>' from -1e6, src/typedef.h:25), the last active pair's interface,
+// and the composition of the prep record (the reference's output composition
+// is the RHS precompute's arithmetic, src/cipher.c:170-200 and 1038-1066)
+template <int G, int NC>
+__global__ __launch_bounds__(256) void k_output(const double *__restrict__ X, const double *__restrict__ cm,
+                                                long long ncells, DevCfg c, OutFields f, double *__restrict__ out) {
+    const long long cell = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int slot = threadIdx.x & (G - 1);
+    if (cell >= ncells) return;
+    const double *x = X + cell * c.D;
+    const int n = clamp_count(x[c.aso], c.P);
+    const bool on = slot < n;
+    const int myid = on ? clamp_id(x[c.aso + 1 + slot], c.npf) : 0;
+    const double phi = on ? x[c.pfo + slot] : 0.0;
+    double hh, h;
+    interpolant<G, double>(c, n, phi, hh, h);
+    double best = -1.0e+6;
+    int bid = 0;
+    #pragma unroll 1
+    for (int g = 0; g < n; g++) {
+        const double hg = gsh<G>(h, g);
+        const int idg = gsh<G>(myid, g);
+        if (hg > best) {
+            best = hg;
+            bid = idg;
+        }
+    }
+    const int ia = gsh<G>(myid, n >= 2 ? n - 2 : 0), ib = gsh<G>(myid, n >= 2 ? n - 1 : 0);
+    // composition of a single-component cell: every site holds the one
+    // component (y = 1), so it is sum_g h_g sum_s nu_s
+    double comp1 = 0.0;
+    if constexpr (NC == 1) {
+        const cipher_material &m = c.mat[c.phase_material[myid]];
+        double sv = 0.0;
+        for (int s = 0; s < m.nsites; s++) sv += h * m.stoichiometry[s];
+        #pragma unroll 1
+        for (int g = 0; g < n; g++) comp1 += gsh<G>(on ? sv : 0.0, g);
+    }
+    for (int k = 0; k < f.n; k++) {
+        const int idx = f.index[k];
+        double v = 0.0;
+        switch (f.kind[k]) {
+        case CIPHER_OUT_PHASEID: v = (double)bid; break;
+        case CIPHER_OUT_MATID: v = (double)c.phase_material[bid]; break;
+        case CIPHER_OUT_INTERFACEID: v = n >= 2 ? (double)c.ilist[ia * c.npf + ib] : -1.0; break;
+        case CIPHER_OUT_CHEMPOT: v = (idx >= 0 && idx < c.ncp - 1) ? x[c.dpo + idx] : 0.0; break;
+        case CIPHER_OUT_COMP:
+            if (idx >= 0 && idx < c.ncp) v = NC > 1 ? cm[cell * (2 * NC + 1) + idx] : comp1;
+            break;
+        case CIPHER_OUT_PHI:
+            #pragma unroll 1
+            for (int g = 0; g < n; g++) {
+                const double pg = gsh<G>(phi, g);
+                if (gsh<G>(myid, g) == idx) v = pg;
+            }
+            break;
+        case CIPHER_OUT_TEMPERATURE: v = x[c.tmo]; break;
+        default: break;
+        }
+        if (slot == 0) out[cell * f.n + k] = v;
+    }
+}
+
+template <int G, int NC>
+static cipher_status output_G(const DevCfg &c, const double *X, double *cm, long long N, const OutFields &f, double *out,
+                              hipStream_t st) {
+    constexpr int CPB = 256 / G;
+    const long long nb = (N + CPB - 1) / CPB;
+    if (nb == 0) return CIPHER_OK;
+    if ((NC > 1 || c.has_tcond) && f.need_comp) {
+        const size_t sh = (size_t)CPB * c.D * sizeof(double);
+        hipLaunchKernelGGL((k_prep<G, NC>), dim3((unsigned)nb), dim3(256), sh, st, X, cm, N, c);
+    }
+    hipLaunchKernelGGL((k_output<G, NC>), dim3((unsigned)nb), dim3(256), 0, st, X, (const double *)cm, N, c, f, out);
+    return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
+}
+
+template <int NC>
+cipher_status launch_output_NC(const DevCfg &c, const double *X, double *cm, long long N, const OutFields &f,
+                               double *out, hipStream_t st) {
+    if (c.P <= 4) return output_G<4, NC>(c, X, cm, N, f, out, st);
+    if (c.P <= 8) return output_G<8, NC>(c, X, cm, N, f, out, st);
+    if (c.P <= 16) return output_G<16, NC>(c, X, cm, N, f, out, st);
+    return CIPHER_ERR_UNSUPPORTED;
+}
+
+// ---------------------------------------------------------------------------
 // interface energy / mobility of a pair from the compact [e0, width, m0]
 // table; an activation factor exp(Q(T)/RT) is evaluated in FP64 (Q/RT is
 // often 10-20, so an FP32 exponent would lose that many ulps)

@@ -3,4 +3,6 @@
 
 namespace cipher {
 template cipher_status launch_NC<3>(const RhsLaunch &L, int precision, hipStream_t st);
+template cipher_status launch_output_NC<3>(const DevCfg &c, const double *X, double *cm, long long N,
+                                             const OutFields &f, double *out, hipStream_t st);
 }  // namespace cipher

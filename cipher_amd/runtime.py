@@ -105,6 +105,21 @@ class Context:
         abi.check(self.lib.cipher_axpy_n(self.h, X.numel(), _ptr(Y), _ptr(X), nk, a, ks), "cipher_axpy_n")
         return Y
 
+    def output(self, X, fields=None):
+        """Derived fields of the output step (src/cipher.c:1005-1080) as an
+        (ncells, nfields) device tensor; fields: (kind, index) pairs or
+        output names, default the config's header/outputs."""
+        from . import config as cfgmod
+        self._check_state(X, "X")
+        if fields is None or (fields and isinstance(fields[0], str)):
+            fields = cfgmod.output_fields(self.cfg, fields)
+        n = len(fields)
+        arr = (C.c_int32 * (2 * max(n, 1)))(*[v for kv in fields for v in kv])
+        out = self.torch.zeros((self.ncells, n), dtype=self.torch.float64, device=self.device)
+        self._stream()
+        abi.check(self.lib.cipher_output(self.h, _ptr(X), n, arr, _ptr(out)), "cipher_output")
+        return out
+
     # ---- launch shapes ------------------------------------------------------
     def shapes(self):
         """(lanes, cap, map) of the five per-size lists."""

@@ -317,6 +317,32 @@ int cipher_ts_accept(cipher_ctx *ctx, double *X_dev);
 int cipher_ts_solve(cipher_ctx *ctx, double *X_dev, double t0, double t_end, double *dt_inout,
                     int64_t max_steps, cipher_ts_stats *stats);
 
+/* ---- derived output fields ---------------------------------------------
+ * The per-cell fields CIPHER writes at an output step (src/cipher.c:1005-1080;
+ * field names of the YAML `outputs` list: "phaseid", "matid", "interfaceid",
+ * "<component>_chempot", "<component>_c", "<phase id>_phi", "temperature"). */
+typedef enum {
+    CIPHER_OUT_PHASEID = 1,     /* id of the phase with the largest interpolant */
+    CIPHER_OUT_MATID = 2,       /* its material */
+    CIPHER_OUT_INTERFACEID = 3, /* interface type of the last active pair, -1 for one phase */
+    CIPHER_OUT_CHEMPOT = 4,     /* chemical potential of component index (< ncp-1) */
+    CIPHER_OUT_COMP = 5,        /* composition of component index (< ncp) */
+    CIPHER_OUT_PHI = 6,         /* phase fraction of phase id index, 0 if inactive */
+    CIPHER_OUT_TEMPERATURE = 7
+} cipher_output_kind;
+
+typedef struct cipher_output_field {
+    int32_t kind, index;
+} cipher_output_field;
+
+#define CIPHER_MAX_OUTPUTS 32
+
+/* out_dev[cell * nfields + f] = field f of cell (nfields <= CIPHER_MAX_OUTPUTS);
+ * a component index out of range gives 0, as an unmatched name leaves the
+ * reference's output untouched. */
+int cipher_output(cipher_ctx *ctx, const double *X_dev, int32_t nfields, const cipher_output_field *fields,
+                  double *out_dev);
+
 /* ---- launch shapes ------------------------------------------------------- */
 /* Shape of one of the five per-size cell lists (list q holds the cells with at
  * most 2^q active phases; all lists run in one launch of 256-thread blocks,

@@ -53,6 +53,7 @@ def lib():
         L.orc_chemenergy_phase.restype = C.c_double
         L.orc_mobility_volumeref.argtypes = [P(abi.Config), vp, vp, vp]
         L.orc_gradients.argtypes = [P(abi.Config), P(abi.Mesh), vp, vp]
+        L.orc_output.argtypes = [P(abi.Config), P(abi.Mesh), vp, C.c_int, vp, vp]
         _lib = L
     return _lib
 
@@ -89,6 +90,19 @@ def setup_problem(cfg, mesh, phase):
     if rc:
         raise RuntimeError(f"orc_setup_problem failed: {rc}")
     return X
+
+
+def output(cfg, mesh, X, fields):
+    """Output fields (src/cipher.c:1005-1080): fields is a list of
+    (kind, index); returns an (ncells, nfields) array."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    fa = np.ascontiguousarray(np.asarray(fields, dtype=np.int32).reshape(-1, 2))
+    out = np.zeros((mesh.ncells, fa.shape[0]), dtype=np.float64)
+    ms = mesh.struct()
+    rc = lib().orc_output(C.byref(cfg.struct), C.byref(ms), _p(X), fa.shape[0], _p(fa), _p(out))
+    if rc:
+        raise RuntimeError(f"orc_output failed: {rc}")
+    return out
 
 
 def wrms_norm(U, Y, atol, rtol):

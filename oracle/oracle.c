@@ -1352,6 +1352,93 @@ int orc_setup_problem(const cipher_config *cfg, const cipher_mesh *mesh, const i
 
 /* TSErrorWeightedNorm 2-norm form (PETSc): sqrt(mean(((|u-y|)/tol)^2)),
  * tol = atol + rtol*max(|u|,|y|). */
+/* Output fields, src/cipher.c:1005-1080: the interpolants of the cell's
+ * active phases, the phase with the largest one (first on ties, `>` against a
+ * running maximum from -LARGE, src/typedef.h:25), its material, the
+ * interface type of the last pair visited, mu, phi of a phase id, T, and the
+ * composition sum_g h_g sum_s nu_s y_{g,s,c} with the site fractions at the
+ * site potentials nu_s (mu_c - mu_int_c) - EX_{g,s,c} (src/cipher.c:1038-1066,
+ * the same arithmetic as the RHS precompute, src/cipher.c:170-200). */
+int orc_output(const cipher_config *cfg, const cipher_mesh *mesh, const double *X, int nfields,
+               const cipher_output_field *fields, double *out)
+{
+    olayout L = orc_layout(cfg);
+    const int N = mesh->ncells, D = L.ndof, nc = cfg->ncp, nd = nc - 1, npf = cfg->npf;
+    const int SF = L.sf > 0 ? L.sf : 1, SP = L.sp;
+    if (L.pf > PMAX || nfields < 0) return CIPHER_ERR_ARG;
+    uint16_t sl[PMAX + 1];
+    double h[PMAX], sf[PMAX * 2 * CIPHER_MAXCP];
+    for (int cell = 0; cell < N; ++cell) {
+        const double *x = X + (size_t)cell * D;
+        f2i(sl, x + L.aso);
+        const double *pc = x + L.pfo, *mu = x + L.dpo, *ex = x + L.exo;
+        const double T = x[L.tmo];
+        orc_interpolant(cfg->interpolation, h, pc, sl[0]);
+        for (int f = 0; f < nfields; f++) {
+            double *o = out + (size_t)cell * nfields + f;
+            const int idx = fields[f].index;
+            double v = 0.0;
+            switch (fields[f].kind) {
+            case CIPHER_OUT_PHASEID:
+            case CIPHER_OUT_MATID: {
+                double mx = -1.0e+6;
+                for (int g = 0; g < sl[0]; g++)
+                    if (h[g] > mx) {
+                        mx = h[g];
+                        v = fields[f].kind == CIPHER_OUT_PHASEID ? (double)sl[g + 1]
+                                                                 : (double)cfg->phase_material[sl[g + 1]];
+                    }
+                break;
+            }
+            case CIPHER_OUT_INTERFACEID:
+                v = -1.0;
+                for (int gk = 0; gk < sl[0]; gk++)
+                    for (int gj = gk + 1; gj < sl[0]; gj++)
+                        v = (double)cfg->interface_list[sl[gk + 1] * npf + sl[gj + 1]];
+                break;
+            case CIPHER_OUT_CHEMPOT:
+                if (idx >= 0 && idx < nd) v = mu[idx];
+                break;
+            case CIPHER_OUT_COMP: {
+                if (idx < 0 || idx >= nc) break;
+                double mui[CIPHER_MAXCP], spim[2 * CIPHER_MAXCP], avg[CIPHER_MAXCP];
+                for (int c = 0; c < nd; c++) mui[c] = 0.0;
+                for (int gk = 0; gk < sl[0]; gk++)
+                    for (int gj = gk + 1; gj < sl[0]; gj++) {
+                        const cipher_interface *it = &cfg->interface[cfg->interface_list[sl[gk + 1] * npf + sl[gj + 1]]];
+                        if (it->has_potential)
+                            for (int c = 0; c < nd; c++) mui[c] += h[gk] * h[gj] * it->potential[c];
+                    }
+                for (int c = 0; c < nc; c++) avg[c] = 0.0;
+                for (int g = 0; g < sl[0]; g++) {
+                    const int mat = cfg->phase_material[sl[g + 1]];
+                    const cipher_material *m = &cfg->material[mat];
+                    for (int s = 0; s < m->nsites; s++)
+                        for (int c = 0; c < nd; c++)
+                            spim[s * nd + c] = m->stoichiometry[s] * (mu[c] - mui[c]) - ex[g * SP + s * nd + c];
+                    orc_sitefrac(cfg, mat, sf + g * SF, spim, T);
+                    for (int s = 0; s < m->nsites; s++)
+                        for (int c = 0; c < nc; c++) avg[c] += h[g] * m->stoichiometry[s] * sf[g * SF + s * nc + c];
+                }
+                v = avg[idx];
+                break;
+            }
+            case CIPHER_OUT_PHI:
+                for (int g = 0; g < sl[0]; g++)
+                    if (sl[g + 1] == idx) v = pc[g];
+                break;
+            case CIPHER_OUT_TEMPERATURE:
+                v = T;
+                break;
+            default:
+                return CIPHER_ERR_ARG;
+            }
+            *o = v;
+        }
+    }
+    return CIPHER_OK;
+}
+
 double orc_wrms_norm(const double *U, const double *Y, int64_t n, double atol, double rtol)
 {
     double sum = 0.0;

@@ -58,6 +58,10 @@ int orc_poststep(const cipher_config *cfg, const cipher_mesh *mesh, double *X);
 int orc_setup_problem(const cipher_config *cfg, const cipher_mesh *mesh,
                       const int32_t *phase, double *X);
 
+/* Output fields of src/cipher.c:1005-1080: out[cell*nfields + f]. */
+int orc_output(const cipher_config *cfg, const cipher_mesh *mesh, const double *X, int nfields,
+               const cipher_output_field *fields, double *out);
+
 /* WRMS error norm of PETSc's TSErrorWeightedNorm (2-norm form). */
 double orc_wrms_norm(const double *U, const double *Y, int64_t n, double atol, double rtol);
 
