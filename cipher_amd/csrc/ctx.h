@@ -17,14 +17,15 @@ struct ShapeStep {
 
 // Steps tried for list q of a mesh with lane capacity G: lanes 2^q..4*2^q
 // (single-phase cells without chemistry keep their one-lane path), both
-// block orders, caps of 4096/16384/65536 blocks.
+// block orders, caps of 16..65536 blocks (a cap below what the list needs
+// makes its blocks stride: slower, never wrong).
 static inline std::vector<ShapeStep> shape_steps(int q, int G, int ncp) {
     std::vector<ShapeStep> out;
     const int base = 1 << q;
     if (!(q == 0 && ncp == 1))
         for (int l = base; l <= std::min(G, base * 4); l *= 2) out.push_back({0, l});
     for (int m : {0, 1}) out.push_back({1, m});
-    for (int cap : {4096, 16384, 65536}) out.push_back({2, cap});
+    for (int cap : {16, 128, 1024, 4096, 16384, 65536}) out.push_back({2, cap});
     return out;
 }
 
