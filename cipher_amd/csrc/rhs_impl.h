@@ -167,22 +167,12 @@ __device__ __forceinline__ void stage_in(double *lds, const double *X, long long
 
 // ---------------------------------------------------------------------------
 // k_prep: composition, mobility, conductivity per cell (src/cipher.c:149-233)
-// (one block of CPB cells; blk is the block's index within the mesh)
+// the prep record of one cell, one group of G lanes (slot = the lane's
+// active slot); x is the cell's record (staged in LDS or in HBM)
 template <int G, int NC>
-__device__ __forceinline__ void prep_cells(const double *__restrict__ X, double *__restrict__ cm, long long ncells,
-                                           const DevCfg &c, long long blk) {
+__device__ __forceinline__ void prep_one(const double *x, long long cell, double *__restrict__ cm, const DevCfg &c,
+                                         int slot) {
     constexpr int ND = NC - 1;
-    constexpr int CPB = 256 / G;
-    extern __shared__ double lds[];
-    const long long c0 = blk * CPB;
-    const int nblk = (int)min((long long)CPB, ncells - c0);
-    stage_in(lds, X, c0, nblk, c.D);
-    __syncthreads();
-    const int lc = threadIdx.x / G;
-    const int slot = threadIdx.x & (G - 1);
-    if (lc >= nblk) return;
-    const long long cell = c0 + lc;
-    const double *x = lds + lc * c.D;
     const int n = clamp_count(x[c.aso], c.P);
     const bool on = slot < n;
     const int myid = on ? clamp_id(x[c.aso + 1 + slot], c.npf) : 0;
@@ -255,6 +245,38 @@ __device__ __forceinline__ void prep_cells(const double *__restrict__ X, double 
             for (int k = 0; k < NC; k++) { o[k] = comp[k]; o[NC + k] = mob[k]; }
         }
         o[2 * NC] = tcond;
+    }
+}
+
+// (one block of CPB cells; blk is the block's index within the mesh)
+template <int G, int NC>
+__device__ __forceinline__ void prep_cells(const double *__restrict__ X, double *__restrict__ cm, long long ncells,
+                                           const DevCfg &c, long long blk) {
+    constexpr int CPB = 256 / G;
+    extern __shared__ double lds[];
+    const long long c0 = blk * CPB;
+    const int nblk = (int)min((long long)CPB, ncells - c0);
+    stage_in(lds, X, c0, nblk, c.D);
+    __syncthreads();
+    const int lc = threadIdx.x / G;
+    const int slot = threadIdx.x & (G - 1);
+    if (lc >= nblk) return;
+    prep_one<G, NC>(lds + lc * c.D, c0 + lc, cm, c, slot);
+}
+
+// the prep records of one per-size list (G lanes per cell, cells read from
+// HBM), blocks blk, blk + nblk, ... of it
+template <int G, int NC>
+__device__ __forceinline__ void prep_list(const double *__restrict__ X, double *__restrict__ cm, long long ncells,
+                                          const DevCfg &c, const int32_t *cells, const int32_t *count, long long blk,
+                                          long long nblk) {
+    const int CPB = blockDim.x / G;
+    const int lc = threadIdx.x / G;
+    const int slot = threadIdx.x & (G - 1);
+    const long long limit = min((long long)*count, ncells);
+    for (long long j = blk * CPB + lc; j < limit; j += nblk * CPB) {
+        const long long cell = cells[j];
+        prep_one<G, NC>(X + cell * c.D, cell, cm, c, slot);
     }
 }
 
@@ -1190,6 +1212,28 @@ __global__ __launch_bounds__(256, NC == 1 ? CIPHER_RHS_MINB1 : CIPHER_RHS_MINBN)
     }
 }
 
+// The prep records of all per-size lists in one launch (block ranges as in
+// k_rhs_all, 2^q lanes per cell of list q), so a cell occupies as many lanes
+// as its active set needs.
+template <int GMAX, int NC>
+__global__ __launch_bounds__(256) void k_prep_all(const double *__restrict__ X, double *__restrict__ cm,
+                                                  long long ncells, DevCfg c, const int32_t *bins,
+                                                  const int32_t *counts, ListRanges rg) {
+    int r = 0;
+    while (r + 1 < rg.n && (int)blockIdx.x >= rg.start[r + 1]) r++;
+    const int q = rg.list[r];
+    const long long blk = (long long)blockIdx.x - rg.start[r], nblk = rg.start[r + 1] - rg.start[r];
+    const int32_t *cells = bins + q * ncells;
+    const int32_t *cnt = counts + q;
+    switch (rg.lanes[r]) {
+    case 1: prep_list<1, NC>(X, cm, ncells, c, cells, cnt, blk, nblk); break;
+    case 2: prep_list<2, NC>(X, cm, ncells, c, cells, cnt, blk, nblk); break;
+    case 4: prep_list<4, NC>(X, cm, ncells, c, cells, cnt, blk, nblk); break;
+    case 8: if constexpr (GMAX >= 8) prep_list<8, NC>(X, cm, ncells, c, cells, cnt, blk, nblk); break;
+    default: if constexpr (GMAX >= 16) prep_list<16, NC>(X, cm, ncells, c, cells, cnt, blk, nblk); break;
+    }
+}
+
 // bin b holds the cells whose active-set size needs 2^b lanes
 __device__ __forceinline__ int lane_bin(int n) {
     return n <= 1 ? 0 : (n == 2 ? 1 : (n <= 4 ? 2 : (n <= 8 ? 3 : 4)));
@@ -1298,10 +1342,6 @@ static cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st)
     if (nb == 0) return CIPHER_OK;
     const DevCfg &c = *L.cfg;
     const bool need_prep = (NC > 1) || c.has_tcond;
-    if (need_prep) {
-        const size_t sh = (size_t)CPB * c.D * sizeof(double);
-        hipLaunchKernelGGL((k_prep<G, NC>), dim3((unsigned)nb), dim3(256), sh, st, L.X, (double *)L.cm, N, c);
-    }
     RhsArgs a;
     a.X = L.X; a.F = L.F; a.active = L.active; a.cellgeom = L.cellgeom;
     a.cell_off = L.cell_off; a.adj = L.adj; a.bcell_off = L.bcell_off; a.bface_id = L.bface_id;
@@ -1319,6 +1359,20 @@ static cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st)
                            nbins, L.bins, N, L.counts);
     } else if (precision == CIPHER_FP32_MIXED) {
         hipLaunchKernelGGL(k_fill_i32<0>, dim3(1), dim3(64), 0, st, L.counts + 8, RHS_COUNTERS - 8, 0);
+    }
+    if (need_prep) {
+        // composition/mobility/conductivity records, list by list with 2^q
+        // lanes per cell (after the lists are known)
+        ListShape ps[5];
+        for (int q = 0; q < 5; q++) {
+            ps[q] = L.shape[q];
+            ps[q].lanes = 0;
+        }
+        size_t l0 = 0;
+        long long tp = 0;
+        const ListRanges pr = make_ranges(ps, N, c, nbins, &l0, &tp);
+        hipLaunchKernelGGL((k_prep_all<G, NC>), dim3((unsigned)tp), dim3(256), 0, st, L.X, (double *)L.cm, N, c,
+                           (const int32_t *)L.bins, (const int32_t *)L.counts, pr);
     }
     size_t lds = 0;
     long long tot = 0;
