@@ -72,15 +72,46 @@ __global__ __launch_bounds__(256, NC == 1 ? CIPHER_RHS_MINB1 : CIPHER_RHS_MINBN)
     }
 }
 
+// the prep records of all lists of all members (the list launch's
+// segments), each cell with its list's lane count
+template <int GMAX, int NC>
+__global__ __launch_bounds__(256) void k_prep_batch_lists(const BatchMember *__restrict__ mem,
+                                                          const int32_t *__restrict__ live,
+                                                          const int32_t *__restrict__ seg,
+                                                          const int32_t *__restrict__ segmem,
+                                                          const int32_t *__restrict__ segq, int nseg, BatchLists bl,
+                                                          const double *X) {
+    const int k = seg_find(seg, nseg, blockIdx.x);
+    const int mi = segmem[k];
+    if (live && !live[mi]) return;
+    const BatchMember &m = mem[mi];
+    const DevCfg c = m.c;
+    if (!(NC > 1 || c.has_tcond)) return;
+    const int q = segq[k];
+    const long long N = m.a.ncells;
+    const int32_t *cells = m.bins + (long long)q * N;
+    const int32_t *cnt = m.counts + q;
+    const double *Xm = X + m.xoff;
+    const long long blk = (long long)blockIdx.x - seg[k], nblk = seg[k + 1] - seg[k];
+    switch (bl.lanes[q]) {
+    case 1: prep_list<1, NC>(Xm, m.cm, N, c, cells, cnt, blk, nblk); break;
+    case 2: prep_list<2, NC>(Xm, m.cm, N, c, cells, cnt, blk, nblk); break;
+    case 4: prep_list<4, NC>(Xm, m.cm, N, c, cells, cnt, blk, nblk); break;
+    case 8: if constexpr (GMAX >= 8) prep_list<8, NC>(Xm, m.cm, N, c, cells, cnt, blk, nblk); break;
+    default: if constexpr (GMAX >= 16) prep_list<16, NC>(Xm, m.cm, N, c, cells, cnt, blk, nblk); break;
+    }
+}
+
 template <int G, int NC>
 static cipher_status batch_G(const BatchLaunch &L, int precision, hipStream_t st) {
-    if (L.prep.nblocks)
-        hipLaunchKernelGGL((k_prep_batch<G, NC>), dim3((unsigned)L.prep.nblocks), dim3(256), L.prep.lds, st, L.mem,
-                           L.live, L.prep.seg, L.prep.segmem, L.prep.nseg, L.X);
     if (L.bin.nblocks)
         hipLaunchKernelGGL(k_bin_batch<0>, dim3((unsigned)L.bin.nblocks), dim3(256), 0, st, L.mem, L.live, L.bin.seg,
                            L.bin.segmem, L.bin.nseg, L.nbins, L.X);
     const BatchSegs &s = L.lists;
+    // prep records after binning, list by list (members that need them)
+    if (L.prep.nblocks && s.nblocks)
+        hipLaunchKernelGGL((k_prep_batch_lists<G, NC>), dim3((unsigned)s.nblocks), dim3(256), 0, st, L.mem, L.live,
+                           s.seg, s.segmem, s.segq, s.nseg, L.shape, L.X);
     if (s.nblocks) {
         if (precision == CIPHER_FP32_MIXED) {
             hipLaunchKernelGGL((k_rhs_batch<G, NC, float>), dim3((unsigned)s.nblocks), dim3(256), s.lds, st, L.mem,
