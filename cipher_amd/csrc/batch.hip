@@ -192,11 +192,11 @@ static int build_tables(cipher_batch *b) {
         long long Nclass = 0;
         for (int i : k.members) Nclass += b->m[i]->ncells;
         const int cpbG = 256 / k.G;
+        // (prep records are written list by list; this table only marks the
+        // members that need them)
         if ((rc = make_segs(b, L.prep, k.members,
                             [&](int i) { return needs_prep(b->m[i]) ? cdiv(b->m[i]->ncells, cpbG) : 0LL; })))
             return rc;
-        L.prep.lds = 0;
-        for (int i : k.members) L.prep.lds = std::max(L.prep.lds, (size_t)cpbG * b->m[i]->L.ndof * sizeof(double));
         if ((rc = make_segs(b, L.bin, k.members, [&](int i) { return cdiv(b->m[i]->ncells, BIN_TILE); }))) return rc;
         if ((rc = make_segs(b, L.post, k.members, [&](int i) { return cdiv(b->m[i]->ncells, cpbG); }))) return rc;
         // all lists of all members in one launch, largest active sets first
@@ -412,11 +412,26 @@ extern "C" int cipher_batch_set_stream(cipher_batch *b, void *stream) {
     return CIPHER_OK;
 }
 
-// the launches of one batched right-hand side, in order, on b->stream
-static int batch_rhs_impl(cipher_batch *b, const double *X, double *F, uint32_t *active, const int32_t *live) {
+// zero the recomputation counters (8..15) of every member, keeping the lists
+__global__ void k_fill_redo(int32_t *counts, int nmem) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nmem * 8) counts[(i / 8) * RHS_COUNTERS + 8 + (i % 8)] = 0;
+}
+
+// the launches of one batched right-hand side, in order, on b->stream; with
+// reuse set, the cells keep the lists of the previous call (stages of one
+// attempt share the active sets)
+static int batch_rhs_impl(cipher_batch *b, const double *X, double *F, uint32_t *active, const int32_t *live,
+                          bool reuse = false) {
     hipStream_t st = b->stream;
     int rc;
-    if ((rc = fill(b, b->d_counts, (long long)RHS_COUNTERS * (long long)b->m.size(), 0, st))) return rc;
+    const int nmem = (int)b->m.size();
+    if (!reuse) {
+        if ((rc = fill(b, b->d_counts, (long long)RHS_COUNTERS * (long long)nmem, 0, st))) return rc;
+    } else if (b->precision == CIPHER_FP32_MIXED) {
+        hipLaunchKernelGGL(k_fill_redo, dim3((unsigned)((nmem * 8 + 255) / 256)), dim3(256), 0, st, b->d_counts, nmem);
+        BCHK(hipGetLastError());
+    }
     b->all.X = X;
     b->all.live = live;
     if (launch_fold_batch(b->all, st)) return cipher_fail(CIPHER_ERR_HIP, "batch fold launch");
@@ -433,6 +448,7 @@ static int batch_rhs_impl(cipher_batch *b, const double *X, double *F, uint32_t 
         k.L.F = F;
         k.L.active = active;
         k.L.live = live;
+        k.L.do_bin = reuse ? 0 : 1;
         hipStream_t s = (ki % 5 == 0) ? st : b->aux[ki % 5 - 1];
         if (launch_rhs_batch(k.L, b->precision, s)) return cipher_fail(CIPHER_ERR_HIP, "batch list launch");
     }
@@ -604,14 +620,15 @@ static int enqueue_attempt(cipher_batch *b, const double *X) {
     const int32_t *live = b->d_live;
     BatchLaunch &A = b->all;
     A.live = live;
+    // stages 2-4 keep the active sets of X: bin once per attempt
     if ((rc = batch_rhs_impl(b, X, s.K[0], nullptr, live))) return rc;
     A.live = live;
     if (launch_axpy_dt_batch(A, s.Y, X, s.dt_dev, 1, A2, k1, st)) return cipher_fail(CIPHER_ERR_HIP, "stage 2");
-    if ((rc = batch_rhs_impl(b, s.Y, s.K[1], nullptr, live))) return rc;
+    if ((rc = batch_rhs_impl(b, s.Y, s.K[1], nullptr, live, true))) return rc;
     if (launch_axpy_dt_batch(A, s.Y, X, s.dt_dev, 1, A3, k2, st)) return cipher_fail(CIPHER_ERR_HIP, "stage 3");
-    if ((rc = batch_rhs_impl(b, s.Y, s.K[2], nullptr, live))) return rc;
+    if ((rc = batch_rhs_impl(b, s.Y, s.K[2], nullptr, live, true))) return rc;
     if (launch_axpy_dt_batch(A, s.Xn, X, s.dt_dev, 3, B, k123, st)) return cipher_fail(CIPHER_ERR_HIP, "stage 4");
-    if ((rc = batch_rhs_impl(b, s.Xn, s.K[3], nullptr, live))) return rc;
+    if ((rc = batch_rhs_impl(b, s.Xn, s.K[3], nullptr, live, true))) return rc;
     if (launch_axpy_dt_batch(A, s.Xh, X, s.dt_dev, 4, BH, k1234, st)) return cipher_fail(CIPHER_ERR_HIP, "embedded");
     A.live = live;
     if (launch_wrms_batch(A, s.Xn, s.Xh, s.p.atol, s.p.rtol, b->d_norms, st)) return cipher_fail(CIPHER_ERR_HIP, "norm");

@@ -158,6 +158,7 @@ struct BatchLaunch {
     const BatchMember *mem;
     const int32_t *live;      // per-member flag (null: all members)
     int ncp, G, nbins;
+    int do_bin;               // sort the cells into lists (0: reuse the previous call's lists)
     BatchSegs fold, prep, bin, lists, post, dofs;
     BatchLists shape;
     const double *X;

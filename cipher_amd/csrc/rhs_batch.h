@@ -12,18 +12,6 @@
 
 namespace cipher {
 
-template <int G, int NC>
-__global__ __launch_bounds__(256) void k_prep_batch(const BatchMember *__restrict__ mem, const int32_t *__restrict__ live,
-                                                    const int32_t *__restrict__ seg, const int32_t *__restrict__ segmem,
-                                                    int nseg, const double *__restrict__ X) {
-    const int k = seg_find(seg, nseg, blockIdx.x);
-    const int mi = segmem[k];
-    if (live && !live[mi]) return;
-    const BatchMember &m = mem[mi];
-    const DevCfg c = m.c;
-    prep_cells<G, NC>(X + m.xoff, m.cm, m.a.ncells, c, (long long)blockIdx.x - seg[k]);
-}
-
 template <int U>
 __global__ __launch_bounds__(256) void k_bin_batch(const BatchMember *__restrict__ mem, const int32_t *__restrict__ live,
                                                    const int32_t *__restrict__ seg, const int32_t *__restrict__ segmem,
@@ -104,7 +92,7 @@ __global__ __launch_bounds__(256) void k_prep_batch_lists(const BatchMember *__r
 
 template <int G, int NC>
 static cipher_status batch_G(const BatchLaunch &L, int precision, hipStream_t st) {
-    if (L.bin.nblocks)
+    if (L.do_bin && L.bin.nblocks)
         hipLaunchKernelGGL(k_bin_batch<0>, dim3((unsigned)L.bin.nblocks), dim3(256), 0, st, L.mem, L.live, L.bin.seg,
                            L.bin.segmem, L.bin.nseg, L.nbins, L.X);
     const BatchSegs &s = L.lists;
