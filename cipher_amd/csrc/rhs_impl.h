@@ -519,17 +519,40 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
             for (int k = kb; k < ke; k++)
                 for (int d = 0; d < c.dim; d++) gr[d] -= c.gw[(long long)k * c.dim + d] * phid;
         }
+        // the face-parallel inputs run ahead of their use: the adjacency
+        // entries two chunks ahead, the neighbour's size, count and first id
+        // one chunk ahead, so a chunk's dependent loads overlap the previous
+        // chunk's work
+        int e_n1 = (kb + slot < ke) ? a.adj[kb + slot] : 0;
+        int e_n2 = (kb + G + slot < ke) ? a.adj[kb + G + slot] : 0;
+        double hN_n = hS, nbnd_n = 0.0, id0d_n = 0.0;
+        if (kb + slot < ke) {
+            const double *xq = a.X + (long long)(e_n1 >> 1) * D;
+            hN_n = a.cellgeom[e_n1 >> 1];
+            nbnd_n = xq[c.aso];
+            id0d_n = xq[c.aso + 1];
+        }
         for (int f0 = kb; f0 < ke; f0 += G) {
             const int nfc = min(G, ke - f0);
             // -- face-parallel part
             const bool fon = slot < nfc;
-            const int e = fon ? a.adj[f0 + slot] : 0;
+            const int e = fon ? e_n1 : 0;
+            const double hN = fon ? hN_n : hS;
+            const double nbnd = fon ? nbnd_n : 0.0;
+            const double id0d = fon ? id0d_n : 0.0;
+            // next chunk: its scalars from the entries loaded one chunk ago,
+            // and the entries of the chunk after it
+            e_n1 = e_n2;
+            if (f0 + G + slot < ke) {
+                const double *xq = a.X + (long long)(e_n1 >> 1) * D;
+                hN_n = a.cellgeom[e_n1 >> 1];
+                nbnd_n = xq[c.aso];
+                id0d_n = xq[c.aso + 1];
+            }
+            e_n2 = (f0 + 2 * G + slot < ke) ? a.adj[f0 + 2 * G + slot] : 0;
             const int nbf = e >> 1;
             const bool selfL = (e & 1) == 0;
             const double *xf = a.X + (long long)nbf * D;
-            const double hN = fon ? a.cellgeom[nbf] : hS;
-            const double nbnd = fon ? xf[c.aso] : 0.0;
-            const double id0d = fon ? xf[c.aso + 1] : 0.0;
             const double dL = selfL ? hS : hN, dR = selfL ? hN : hS;
             const double vL = dfastpow(dL, (unsigned)dim), vR = dfastpow(dR, (unsigned)dim);
             const double ffd = 2.0 * (dL < dR ? vL / dL : vR / dR) / (dL + dR);
