@@ -1,31 +1,33 @@
 // rhs_impl.h - the phase-field right-hand side on gfx950 (kernels and launch).
 //
-// Restates RHSFunction of micmog/CIPHER (src/cipher.c:27-681) as two kernels:
+// Restates RHSFunction of micmog/CIPHER (src/cipher.c:27-681).  One RHS is
 //
-//   k_prep  per-cell composition, volume-ref mobility and thermal
-//           conductivity (src/cipher.c:149-233) into a compact FP64 record,
-//           only when the face fluxes need them (chemistry or conduction);
-//   k_rhs   per-cell gather over the cell's faces in face order (the
-//           reference scatters faces in the same order, so each cell sums its
-//           Laplacian contributions in the same sequence, src/cipher.c:236-389)
-//           fused with the cell update (src/cipher.c:392-664).
+//   k_bin       the cells sorted into five lists by active-set size n;
+//   k_prep_all  per-cell composition, volume-ref mobility and thermal
+//               conductivity (src/cipher.c:149-233) into a compact FP64
+//               record, only when the face fluxes need them (chemistry or
+//               conduction), list by list;
+//   k_rhs_all   per-cell gather over the cell's faces in face order (the
+//               reference scatters faces in the same order, so each cell
+//               sums its Laplacian contributions in the same sequence,
+//               src/cipher.c:236-389) fused with the cell update
+//               (src/cipher.c:392-664); all lists in one launch, one block
+//               range per list.
 //
-// Mapping: the cells are first sorted into lists by active-set size n
-// (k_bin); each list is processed with G = next power of two >= n lanes per
-// cell, lane g owning active slot g, so a two-phase cell takes two lanes
-// whatever PF_SIZE is.  Per-cell sums over slots are replayed by every lane
-// in the reference's loop order through group shuffles, so FP64 results
-// follow the reference's rounding sequence.  A group stages its cell's record
-// in LDS and writes the cell's F record.  Neighbour records are fetched a
-// batch of faces at a time so the dependent loads of several faces are in
-// flight together.
+// Mapping: list q is processed with G = 2^q lanes per cell (or more, a
+// launch-shape choice), lane g owning active slot g, so a two-phase cell
+// takes two lanes whatever PF_SIZE is.  Per-cell sums over slots are replayed
+// by every lane in the reference's loop order through group shuffles, so
+// FP64 results follow the reference's rounding sequence.  Faces are handled G
+// at a time; the loads of the next chunk's faces run ahead of their use.
 //
-// FP32-mixed: the face fluxes (Kahan-compensated), interpolants, capillary and
-// rate assembly run in FP32; the implicit site potentials, site fractions,
-// chemical energy, explicit potentials and their rate stay FP64.  Every slot
-// whose active flag is decided by the sign of an FP32 value lying inside its
-// rounding band sends its cell to a list that the FP64 kernel recomputes, so
-// the active sets equal the FP64 path's.
+// FP32-mixed: the phase Laplacian (Kahan-compensated), interpolants,
+// capillary terms and pair rates run in FP32; chemical fluxes, site
+// potentials, site fractions, chemical energy, driving force, explicit
+// potentials and the mu rate stay FP64.  Every cell with a slot whose active
+// flag is decided by the sign of an FP32 value inside its rounding band goes
+// to its list's recomputation list, which an FP64 launch recomputes, so the
+// active sets equal the FP64 path's.
 #pragma once
 #include <algorithm>
 
