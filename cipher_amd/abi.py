@@ -163,6 +163,8 @@ def _declare(lib):
         "cipher_ts_attempt": (i32, [vp, vp, dbl, C.POINTER(C.c_double)]),
         "cipher_ts_accept": (i32, [vp, vp]),
         "cipher_ts_solve": (i32, [vp, vp, dbl, dbl, C.POINTER(C.c_double), i64, C.POINTER(TsStats)]),
+        "cipher_ts_loadcases": (i32, [vp, vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_double), dbl,
+                                      C.POINTER(TsStats)]),
         "cipher_batch_create": (i32, [C.POINTER(vp), i32, C.POINTER(vp)]),
         "cipher_batch_destroy": (i32, [vp]),
         "cipher_batch_offsets": (i32, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
@@ -195,6 +197,7 @@ EXPORTED = ("cipher_layout_of", "cipher_ctx_create", "cipher_ctx_destroy", "ciph
             "cipher_upload_mesh", "cipher_rhs", "cipher_rhs_active", "cipher_poststep",
             "cipher_setup_problem", "cipher_wrms_norm", "cipher_axpy_n", "cipher_last_error",
             "cipher_ts_adapt", "cipher_ts_setup", "cipher_ts_attempt", "cipher_ts_accept", "cipher_ts_solve",
+            "cipher_ts_loadcases",
             "cipher_batch_create", "cipher_batch_destroy", "cipher_batch_offsets", "cipher_batch_set_precision",
             "cipher_batch_set_stream", "cipher_batch_rhs", "cipher_batch_rhs_active", "cipher_batch_poststep",
             "cipher_batch_wrms_norm", "cipher_batch_autotune", "cipher_batch_shape", "cipher_batch_ts_setup",

@@ -253,3 +253,29 @@ extern "C" int cipher_ts_solve(cipher_ctx *c, double *X, double t0, double t_end
     *dt_inout = dt;
     return rc;
 }
+
+// Load cases, src/cipher.c:1589-1606: the case's end time is the running
+// sum loadcasetime + time[k], each case restarts at timestepmin.  The
+// reference splits a case into TSSolve calls of amrinterval steps and re-reads
+// the step size between them; without remeshing that is one solve.
+extern "C" int cipher_ts_loadcases(cipher_ctx *c, double *X, int32_t ncases, const double *times,
+                                   const double *rates, double t0, cipher_ts_stats *stats) {
+    if (!c || !X || !times || !stats || ncases < 0) return cipher_fail(CIPHER_ERR_ARG, "bad argument");
+    if (!c->ts.ready) return cipher_fail(CIPHER_ERR_STATE, "cipher_ts_setup not called for this mesh");
+    for (int32_t k = 0; k < ncases; k++)
+        if (!(times[k] >= 0.0)) return cipher_fail(CIPHER_ERR_ARG, "load-case time must be >= 0");
+    double start = t0, t = t0;
+    for (int32_t k = 0; k < ncases; start += times[k++]) {
+        int rc = cipher_set_temperature_rate(c, rates != nullptr, rates ? rates[k] : 0.0);
+        if (rc) return rc;
+        const double end = start + times[k];
+        double dt = c->ts.p.dt_min;
+        cipher_ts_stats st{};
+        st.t = t;
+        st.dt = dt;
+        if (t < end && (rc = cipher_ts_solve(c, X, t, end, &dt, 0, &st))) return rc;
+        t = st.t;
+        stats[k] = st;
+    }
+    return CIPHER_OK;
+}

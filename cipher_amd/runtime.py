@@ -171,6 +171,25 @@ class Context:
                   "cipher_ts_solve")
         return st, d.value
 
+    def solve_loadcases(self, X, times=None, rates=(), t0=0.0):
+        """CIPHER's load-case loop (src/cipher.c:1589-1606) on X in place:
+        times and rates default to the configuration's solution_parameters
+        time and temperature_rate lists (rates None: no temperature rate).
+        Returns one TsStats per load case."""
+        self._check_state(X, "X")
+        times = list(self.cfg.times if times is None else times)
+        if rates == ():
+            rates = self.cfg.temperature_rates
+        if rates is not None and len(rates) != len(times):
+            raise ValueError("one temperature rate per load case")
+        n = len(times)
+        ta = (C.c_double * max(n, 1))(*times)
+        ra = (C.c_double * max(n, 1))(*rates) if rates is not None else None
+        st = (abi.TsStats * max(n, 1))()
+        self._stream()
+        abi.check(self.lib.cipher_ts_loadcases(self.h, _ptr(X), n, ta, ra, t0, st), "cipher_ts_loadcases")
+        return list(st)[:n]
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.cipher_ctx_destroy(self.h)

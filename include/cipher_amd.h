@@ -317,6 +317,19 @@ int cipher_ts_accept(cipher_ctx *ctx, double *X_dev);
 int cipher_ts_solve(cipher_ctx *ctx, double *X_dev, double t0, double t_end, double *dt_inout,
                     int64_t max_steps, cipher_ts_stats *stats);
 
+/* The load-case loop of CIPHER's main (src/cipher.c:1589-1606; cases from
+ * solution_parameters/time and temperature_rate, src/init.c:1248-1256): for
+ * load case k = 0..ncases-1, set the temperature rate rates[k] (rates NULL:
+ * no temperature rate in any case, the reference's NULL array), restart from
+ * the step size params.dt_min (timestepmin) and integrate with an exact final
+ * time to T_k = (...((t0 + times[0]) + times[1]) ...) + times[k], the
+ * reference's running sum.  The step controller's memory carries over from
+ * case to case as it does between cipher_ts_solve calls.  stats[k] receives
+ * case k's counts and the time reached.  Leaves the last case's temperature
+ * rate set. */
+int cipher_ts_loadcases(cipher_ctx *ctx, double *X_dev, int32_t ncases, const double *times,
+                        const double *rates, double t0, cipher_ts_stats *stats);
+
 /* ---- derived output fields ---------------------------------------------
  * The per-cell fields CIPHER writes at an output step (src/cipher.c:1005-1080;
  * field names of the YAML `outputs` list: "phaseid", "matid", "interfaceid",

@@ -1274,7 +1274,9 @@ int orc_poststep(const cipher_config *cfg, const cipher_mesh *mesh, double *X)
         }
         if (rc != CIPHER_OK) break;
         if (ss[0] >= L.as) { rc = CIPHER_ERR_CAPACITY; break; }
-        double phiss[PMAX], spss[PMAX * 2 * CIPHER_MAXCP];
+        /* zeroed: a one-sublattice material fills only the first nd entries of
+         * its slot; the rest are never read, and stay deterministic */
+        double phiss[PMAX], spss[PMAX * 2 * CIPHER_MAXCP] = {0};
         for (int g = 0; g < ss[0]; g++) {
             phiss[g] = 0.0;
             int mat = cfg->phase_material[ss[g + 1]];

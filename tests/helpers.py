@@ -286,12 +286,13 @@ def cpu_attempt(wl, X, dt, atol, rtol, rhs=None):
     return Xn, Xh, oracle.wrms_norm(Xn, Xh, atol, rtol)
 
 
-def cpu_solve(wl, X, t0, t_end, dt, params, max_steps=0):
+def cpu_solve(wl, X, t0, t_end, dt, params, max_steps=0, ctrl=None):
     """The device integrator's loop (exact final time, PostStep after each
-    accepted step) on the CPU, with the library's host step controller."""
+    accepted step) on the CPU, with the library's host step controller;
+    ctrl carries the controller's memory in and out (fresh if None)."""
     from cipher_amd import abi
     from cipher_amd.runtime import ts_adapt
-    ctrl = abi.TsCtrl()
+    ctrl = abi.TsCtrl() if ctrl is None else ctrl
     t, steps, rejected, dts = t0, 0, 0, []
     X = np.array(X, dtype=np.float64)
     while t < t_end and (max_steps <= 0 or steps < max_steps):
@@ -310,6 +311,32 @@ def cpu_solve(wl, X, t0, t_end, dt, params, max_steps=0):
             rejected += 1
         dt = dt_next
     return X, dict(steps=steps, rejected=rejected, t=t, dt=dt, history=dts)
+
+
+def cpu_loadcases(wl, X, times, rates, params, t0=0.0):
+    """CIPHER's load-case loop (src/cipher.c:1589-1606) over cpu_solve: case
+    k sets the temperature rate rates[k] (none if rates is None), restarts
+    from dt = params.dt_min and runs to the running sum t0 + times[0] + ...
+    + times[k]; one controller memory for the whole run.  Restores the
+    configuration's rate afterwards."""
+    from cipher_amd import abi
+    s = wl.cfg.struct
+    saved = (s.has_temperature_rate, s.temperature_rate)
+    ctrl, out, start, t = abi.TsCtrl(), [], t0, t0
+    try:
+        for k, tau in enumerate(times):
+            s.has_temperature_rate = 0 if rates is None else 1
+            s.temperature_rate = 0.0 if rates is None else rates[k]
+            end = start + tau
+            info = dict(steps=0, rejected=0, t=t, dt=params.dt_min)
+            if t < end:
+                X, info = cpu_solve(wl, X, t, end, params.dt_min, params, ctrl=ctrl)
+            t = info["t"]
+            out.append(info)
+            start += tau
+    finally:
+        s.has_temperature_rate, s.temperature_rate = saved
+    return X, out
 
 
 GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.abspath(__file__)), "golden")

@@ -102,6 +102,27 @@ def test_bogacki_shampine_orders():
     assert 12.0 < r_loc < 20.0, loc
 
 
+def test_loadcases_integrate_the_temperature_rate():
+    """Load cases (src/cipher.c:1589-1606) on the oracle: with no conduction
+    the temperature integrates the piecewise constant rate exactly, each
+    case ends at the running sum of the case lengths, and a case set to the
+    configuration's own state (no rate) follows cpu_solve step for step."""
+    wl = H.chem()
+    p = abi.ts_params(1e-4, 1e-4, 1e-6, 1e-3, use_graph=False)
+    X0 = H.initial_state(wl)
+    L = wl.layout
+    times, rates = [4e-4, 3e-4], [1.0e4, -2.0e4]
+    X, infos = H.cpu_loadcases(wl, X0, times, rates, p)
+    assert [i["t"] for i in infos] == [4e-4, 4e-4 + 3e-4]
+    T0 = X0.reshape(-1, L["ndof"])[:, L["tm_offset"]]
+    T = X.reshape(-1, L["ndof"])[:, L["tm_offset"]]
+    np.testing.assert_allclose(T, T0 + 1.0e4 * 4e-4 - 2.0e4 * 3e-4, rtol=1e-12)
+    assert wl.cfg.struct.has_temperature_rate == 0      # restored
+    X1, i1 = H.cpu_loadcases(wl, X0, [7e-4], None, p)
+    X2, i2 = H.cpu_solve(wl, X0, 0.0, 7e-4, p.dt_min, p)
+    assert np.array_equal(X1, X2) and i1[0]["steps"] == i2["steps"]
+
+
 pytestmark_gpu = pytest.mark.gpu
 
 
