@@ -443,10 +443,13 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
             // zero phase rate and Laplacian, so F is 0 except the imposed
             // temperature rate (src/cipher.c:558-568).  A wave writes its 64
             // cells' records together: one contiguous range when the cells
-            // are consecutive, else one record per step (D lanes, two cache
-            // lines) rather than one double per lane per step (64 lines).
+            // are consecutive, else as many whole records per step as the
+            // wave's lanes cover.  Records of an even length D start on 16
+            // bytes (F 16-byte aligned), so they are written as double2.
             const int lane = (int)(threadIdx.x & 63);
             const double tr = c.has_trate ? 0.0 + c.trate : 0.0;
+            const int D2 = D >> 1;
+            const bool wide = (D & 1) == 0 && D2 <= 64 && ((uintptr_t)a.F & 15) == 0;
             for (long long j0 = blk * CPB + (lc & ~63); j0 < limit; j0 += stride) {
                 const long long j = j0 + lane;
                 const int cell = j < limit ? a.cells[j] : -1;
@@ -454,7 +457,33 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                 const int c0 = __shfl(cell, 0, 64);
                 const int cnt = (int)min((long long)64, limit - j0);
                 const bool contig = __all(cell < 0 || cell == c0 + lane);
-                if (contig) {
+                if (wide) {
+                    if (contig) {
+                        double2 *F0 = reinterpret_cast<double2 *>(a.F + (long long)c0 * D);
+                        const int tot = cnt * D2;
+                        const int step = 64 % D2;
+                        int rem = lane % D2;           // double2 index within the record
+                        for (int k = lane; k < tot; k += 64) {
+                            double2 v = {0.0, 0.0};
+                            if (c.has_trate && 2 * rem == c.tmo) v.x = tr;
+                            if (c.has_trate && 2 * rem + 1 == c.tmo) v.y = tr;
+                            F0[k] = v;
+                            rem += step;
+                            if (rem >= D2) rem -= D2;
+                        }
+                    } else {
+                        const int ppr = 64 / D2;           // records per step
+                        const int r = lane / D2, f = lane - r * D2;
+                        double2 v = {0.0, 0.0};
+                        if (c.has_trate && 2 * f == c.tmo) v.x = tr;
+                        if (c.has_trate && 2 * f + 1 == c.tmo) v.y = tr;
+                        for (int i0 = 0; i0 < cnt; i0 += ppr) {
+                            const int i = i0 + r;
+                            const int ci = __shfl(cell, min(i, 63), 64);
+                            if (r < ppr && i < cnt) reinterpret_cast<double2 *>(a.F + (long long)ci * D)[f] = v;
+                        }
+                    }
+                } else if (contig) {
                     double *F0 = a.F + (long long)c0 * D;
                     const int tot = cnt * D;
                     const int step = 64 % D;
@@ -1265,16 +1294,18 @@ __device__ __forceinline__ int lane_bin(int n) {
 }
 
 // Sort the cells into per-size lists.  A block takes a tile of BIN_TILE
-// consecutive cells, each thread BIN_CPT consecutive cells of it; a block scan
-// of the per-thread counts gives every cell its place, and one atomic per bin
-// per tile reserves the tile's range, so each list keeps cell order within a
-// tile (and the Morton locality of the tiles).
+// consecutive cells, each thread BIN_CPT consecutive cells of it; a scan of
+// the per-thread counts (shuffles within each wave, then the four wave totals)
+// gives every cell its place, and one atomic per bin per tile reserves the
+// tile's range, so each list keeps cell order within a tile (and the Morton
+// locality of the tiles).  Needs blockDim.x == 256.
 __device__ __forceinline__ void bin_tile(const double *__restrict__ X, long long ncells, const DevCfg &c, int nbins,
                                          int32_t *__restrict__ bins, long long binstride,
                                          int32_t *__restrict__ counts, long long blk) {
-    __shared__ int scan[5][256];
-    __shared__ int base[5];
+    __shared__ int wtot[4][5];
+    __shared__ int base[4][5];
     const long long c0 = blk * BIN_TILE + (long long)threadIdx.x * BIN_CPT;
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
     int bq[BIN_CPT];
     int cnt[5] = {0, 0, 0, 0, 0};
 #pragma unroll
@@ -1286,27 +1317,37 @@ __device__ __forceinline__ void bin_tile(const double *__restrict__ X, long long
     for (int k = 0; k < BIN_CPT; k++)
 #pragma unroll
         for (int b = 0; b < 5; b++) cnt[b] += (bq[k] == b) ? 1 : 0;
+    // inclusive scan over the wave's lanes, per bin
+    int inc[5];
 #pragma unroll
-    for (int b = 0; b < 5; b++) scan[b][threadIdx.x] = cnt[b];
-    __syncthreads();
-    // inclusive Hillis-Steele scan of the 256 per-thread counts, per bin
-    for (int off = 1; off < 256; off <<= 1) {
-        int v[5];
+    for (int b = 0; b < 5; b++) inc[b] = cnt[b];
 #pragma unroll
-        for (int b = 0; b < 5; b++) v[b] = threadIdx.x >= (unsigned)off ? scan[b][threadIdx.x - off] : 0;
-        __syncthreads();
+    for (int off = 1; off < 64; off <<= 1) {
 #pragma unroll
-        for (int b = 0; b < 5; b++) scan[b][threadIdx.x] += v[b];
-        __syncthreads();
+        for (int b = 0; b < 5; b++) {
+            const int v = __shfl_up(inc[b], (unsigned)off, 64);
+            if (lane >= off) inc[b] += v;
+        }
     }
+    if (lane == 63) {
+#pragma unroll
+        for (int b = 0; b < 5; b++) wtot[wave][b] = inc[b];
+    }
+    __syncthreads();
     if (threadIdx.x < (unsigned)nbins) {
-        const int tot = scan[threadIdx.x][255];
-        base[threadIdx.x] = tot ? atomicAdd(&counts[threadIdx.x], tot) : 0;
+        const int b = (int)threadIdx.x;
+        const int t0 = wtot[0][b], t1 = wtot[1][b], t2 = wtot[2][b], t3 = wtot[3][b];
+        const int tot = t0 + t1 + t2 + t3;
+        const int g = tot ? atomicAdd(&counts[b], tot) : 0;
+        base[0][b] = g;
+        base[1][b] = g + t0;
+        base[2][b] = g + t0 + t1;
+        base[3][b] = g + t0 + t1 + t2;
     }
     __syncthreads();
     int pos[5];
 #pragma unroll
-    for (int b = 0; b < 5; b++) pos[b] = base[b] + scan[b][threadIdx.x] - cnt[b];
+    for (int b = 0; b < 5; b++) pos[b] = (b < nbins ? base[wave][b] : 0) + inc[b] - cnt[b];
 #pragma unroll
     for (int k = 0; k < BIN_CPT; k++) {
         const int b = bq[k];

@@ -117,6 +117,35 @@ def test_rhs_grain_growth_bit_exact():
     assert np.array_equal(Fg, Fo), f"max diff {np.abs(Fg - Fo).max():.3e}"
 
 
+@pytest.mark.parametrize("name", ["gg2d", "gg3d_amr"])
+@pytest.mark.parametrize("aligned", [True, False], ids=["F16", "F8"])
+def test_rhs_grain_growth_single_phase_cells_with_temperature_rate(name, aligned):
+    """The single-phase write path of grain growth (whole records per wave,
+    16-byte stores when F is 16-byte aligned, 8-byte ones otherwise; runs of
+    consecutive cells and scattered cells) with an imposed temperature rate:
+    identical bits to the oracle, every cell's T rate equal to the rate."""
+    import torch
+    wl = WORKLOADS[name]()
+    X = H.perturb(wl, H.evolve(wl, H.initial_state(wl), steps=4))
+    s = wl.cfg.struct
+    saved = (s.has_temperature_rate, s.temperature_rate)
+    try:
+        s.has_temperature_rate, s.temperature_rate = 1, -3.25
+        Fo = oracle.rhs(wl.cfg, wl.mesh, X)
+    finally:
+        s.has_temperature_rate, s.temperature_rate = saved
+    ctx = _ctx(wl)
+    abi.check(ctx.lib.cipher_set_temperature_rate(ctx.h, 1, -3.25), "cipher_set_temperature_rate")
+    buf = torch.full((X.size + 1,), float("nan"), dtype=torch.float64, device=ctx.device)
+    F = buf[:X.size] if aligned else buf[1:]
+    Fg = _host(ctx.rhs(_dev(ctx, X), F=F))
+    D = wl.layout["ndof"]
+    print(f"[single-phase] {name} aligned={aligned} single-phase cells "
+          f"{int((X.reshape(-1, D)[:, 0] == 1).sum())} of {wl.ncells}", flush=True)
+    assert np.array_equal(Fg, Fo), f"max diff {np.nanmax(np.abs(Fg - Fo)):.3e}"
+    assert np.all(Fg.reshape(-1, D)[:, wl.layout["tm_offset"]] == -3.25)
+
+
 @pytest.mark.parametrize("name", list(WORKLOADS))
 def test_poststep_matches_oracle(name):
     wl = WORKLOADS[name]()
