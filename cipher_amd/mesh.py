@@ -11,7 +11,11 @@ builds the equivalent on the host with numpy:
 * ``adapt_to_labels``: top-down refinement at label interfaces between the
   minimum and maximum levels, followed by ripple 2:1 balancing;
 * ``Forest.flatten``: the face list (L = low side, R = high side of the face),
-  boundary faces with the label ids of src/cipher.c:1214-1220, and h.
+  boundary faces with the label ids of src/cipher.c:1214-1220, and h;
+* ``adapt`` / ``transfer``: the time loop's remesh (src/cipher.c:1607-1700):
+  refine and coarsen by label within the level limits, 2:1 balance, and the
+  state carried over (copied into refined leaves, averaged into coarsened
+  ones).
 
 Faces are ordered by (L, direction, R); cell ordering and face ordering are
 this module's own (DMPlex numbering is not reproduced).
@@ -296,6 +300,77 @@ def adapt_to_labels(labels, blocks, size, periodic, minlevel, maxlevel):
             break
     f = Forest(res, blocks, size, periodic, np.concatenate(leaves_l), np.concatenate(leaves_a))
     return balance(f)
+
+
+def adapt(forest, refine, minlevel, maxlevel):
+    """DMAdaptLabel on the forest (src/cipher.c:1616-1668, level limits from
+    DMForestSetMinimum/MaximumRefinement, src/cipher.c:1153-1154): a leaf
+    labelled for refinement (refine[i] True) is split once if it is below
+    maxlevel; every other leaf carries the label's default, coarsen, and a
+    family of 2^dim sibling leaves that all carry it is merged into its
+    parent if that stays at or above minlevel; then 2:1 balance.  One level
+    per call either way, as p4est's non-recursive refine and coarsen."""
+    f = forest
+    dim = f.dim
+    refine = np.asarray(refine, dtype=bool)
+    if refine.shape != (f.ncells,):
+        raise ValueError("one refinement flag per leaf")
+    ext = f.cell_extent()
+    # families: leaves of one level sharing the parent anchor
+    pext = 2 * ext
+    parent = (f.anchor // pext[:, None]) * pext[:, None]
+    bits = max(1, int(math.ceil(math.log2(max(int(f.res.max()), 2)))))
+    key = morton_keys(parent, bits).astype(np.int64) * 64 + f.level
+    coarsen = (f.level > minlevel) & ~refine
+    uk, inv, cnt = np.unique(key, return_inverse=True, return_counts=True)
+    ok = np.ones(uk.size, dtype=bool)
+    np.logical_and.at(ok, inv, coarsen)
+    merge = ok[inv] & (cnt[inv] == 1 << dim)
+    split = refine & (f.level < maxlevel)
+    keep = ~merge & ~split
+    lv = [f.level[keep]]
+    an = [f.anchor[keep]]
+    if split.any():
+        offs = np.array(np.meshgrid(*[[0, 1]] * dim, indexing="ij")).reshape(dim, -1).T
+        half = ext[split] // 2
+        an.append((f.anchor[split][:, None, :] + offs[None, :, :] * half[:, None, None]).reshape(-1, dim))
+        lv.append(np.repeat(f.level[split] + 1, offs.shape[0]))
+    if merge.any():
+        first = merge & np.all(f.anchor == parent, axis=1)      # one entry per merged family
+        an.append(parent[first])
+        lv.append(f.level[first] - 1)
+    g = Forest(f.res, f.blocks, f.size, f.periodic, np.concatenate(lv), np.concatenate(an))
+    return balance(g)
+
+
+def transfer(old, new, X):
+    """DMForestTransferVec (src/cipher.c:1675) of cell records X (old.ncells
+    rows): a new leaf inside an old leaf takes its record (refinement
+    copies); a new leaf covering old leaves takes their mean, summed over
+    the children in (x, y, z) index order, z fastest (coarsening averages
+    equal volumes)."""
+    X = np.asarray(X)
+    if X.shape[0] != old.ncells:
+        raise ValueError("X must have one row per old leaf")
+    dim = new.dim
+    idm = old.id_map()
+    cover = idm[tuple(new.anchor.T)]
+    out = np.empty((new.ncells,) + X.shape[1:], dtype=X.dtype)
+    same = old.level[cover] <= new.level
+    out[same] = X[cover[same]]
+    rest = np.flatnonzero(~same)
+    if rest.size:
+        offs = np.array(np.meshgrid(*[[0, 1]] * dim, indexing="ij")).reshape(dim, -1).T
+        half = new.cell_extent()[rest] // 2
+        ch = new.anchor[rest][:, None, :] + offs[None, :, :] * half[:, None, None]
+        ids = idm[tuple(ch.reshape(-1, dim).T)].reshape(rest.size, -1)
+        if np.any(old.level[ids] != (new.level[rest] + 1)[:, None]):
+            raise ValueError("a coarsened leaf must cover one family of old leaves")
+        acc = X[ids[:, 0]].copy()
+        for k in range(1, ids.shape[1]):
+            acc += X[ids[:, k]]
+        out[rest] = acc / ids.shape[1]
+    return out
 
 
 def cell_labels(forest, labels):

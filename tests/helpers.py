@@ -23,6 +23,18 @@ def gg3d_amr(res=(16, 16, 16), ngrains=5, minlevel=2):
     return _cache[key]
 
 
+def gg2d_amr(res=(64, 64), ngrains=6, minlevel=3, amrinterval=None):
+    """2D periodic grain growth on an adaptive quadtree (levels minlevel..6)."""
+    key = ("gg2d_amr", res, ngrains, minlevel, amrinterval)
+    if key not in _cache:
+        txt = W.grain_growth_yaml(res=res, ngrains=ngrains, periodic=True, minlevel=minlevel)
+        if amrinterval is not None:
+            assert "amrinterval : 1000000\n" in txt
+            txt = txt.replace("amrinterval : 1000000\n", f"amrinterval : {amrinterval}\n")
+        _cache[key] = W.Workload(txt, "gg2d_amr")
+    return _cache[key]
+
+
 def ternary(res=(16, 16, 16), ngrains=4, nppt=2, radius=2, minlevel=2):
     key = ("tern", res, ngrains, nppt, radius, minlevel)
     if key not in _cache:
@@ -337,6 +349,49 @@ def cpu_loadcases(wl, X, times, rates, params, t0=0.0):
     finally:
         s.has_temperature_rate, s.temperature_rate = saved
     return X, out
+
+
+def cpu_timeloop(cfg, forest, X, params, times, rates, interval, t0=0.0):
+    """CIPHER's time loop with remeshing (src/cipher.c:1589-1830) over
+    cpu_solve, in the arithmetic of cipher_amd.timeloop.solve: chunks end at
+    multiples of interval accepted steps or at the case's end; after a chunk
+    ending on a multiple the host remesh (timeloop.adapt_state) runs and, if
+    the forest changed, the controller memory starts afresh (a new TS).
+    Returns (forest, X, chunks, remeshes)."""
+    import types
+    from cipher_amd import abi
+    from cipher_amd import timeloop as TL
+    s = cfg.struct
+    saved = (s.has_temperature_rate, s.temperature_rate)
+    wl = types.SimpleNamespace(cfg=cfg, mesh=forest.flatten())
+    ctrl, chunks, remeshes = abi.TsCtrl(), [], 0
+    current = case_t = t0
+    nsteps = 0
+    X = np.array(X, dtype=np.float64)
+    try:
+        for k, tau in enumerate(times):
+            s.has_temperature_rate = 0 if rates is None else 1
+            s.temperature_rate = 0.0 if rates is None else rates[k]
+            dt = params.dt_min
+            end = case_t + tau
+            while current < end:
+                X, info = cpu_solve(wl, X, current, end, dt, params, max_steps=interval - nsteps % interval,
+                                    ctrl=ctrl)
+                assert info["steps"] > 0
+                current, dt = info["t"], info["dt"]
+                nsteps += info["steps"]
+                chunks.append((k, info["steps"], info["rejected"], current, dt, wl.mesh.ncells))
+                if nsteps % interval == 0:
+                    f2, X2 = TL.adapt_state(forest, X, cfg.layout(), cfg.minnrefine, cfg.maxnrefine)
+                    if f2 is not forest:
+                        forest, X = f2, np.array(X2)
+                        wl.mesh = forest.flatten()
+                        ctrl = abi.TsCtrl()
+                        remeshes += 1
+            case_t += tau
+    finally:
+        s.has_temperature_rate, s.temperature_rate = saved
+    return forest, X, chunks, remeshes
 
 
 GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.abspath(__file__)), "golden")

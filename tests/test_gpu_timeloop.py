@@ -1,0 +1,65 @@
+"""CIPHER's time loop with remeshing (src/cipher.c:1589-1830) through
+cipher_amd.timeloop.solve on the device against its CPU restatement
+(tests/helpers.py cpu_timeloop: oracle RHS and PostStep, the same host
+remesh), on 2D periodic grain growth on an adaptive quadtree (levels 3-6,
+1660 leaves at the start) with two load cases of different temperature
+rates and a remesh every 10 accepted steps (37 remeshes, 1660 -> ~2200
+leaves): every chunk's case, accepted and rejected counts and leaf count
+equal, times and next step sizes within 1e-10, the same final forest, the
+same active sets and phase fractions within 1e-9, and the temperature
+T0 + sum_k rate_k * time_k in every leaf (refinement copies it, coarsening
+averages equal values).
+"""
+import numpy as np
+import pytest
+
+from cipher_amd import abi
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+TIMES = [20.0, 10.0]
+RATES = [0.5, -0.5]
+INTERVAL = 10
+
+
+def _params(wl, graph):
+    return abi.ts_params(1e-4, 1e-4, wl.cfg.timestepmin, 100.0, use_graph=graph)
+
+
+def test_timeloop_with_remeshing_matches_cpu():
+    import torch
+    from cipher_amd.runtime import Context
+    from cipher_amd import timeloop as TL
+    wl = H.gg2d_amr(amrinterval=INTERVAL)
+    print(f"[timeloop] {wl.ncells} leaves, cases {TIMES} rates {RATES}, remesh every {INTERVAL} steps", flush=True)
+    fc, Xc, chunks_c, rem_c = H.cpu_timeloop(wl.cfg, wl.forest, H.initial_state(wl), _params(wl, False),
+                                             TIMES, RATES, INTERVAL)
+    print(f"   cpu: {len(chunks_c)} chunks, {rem_c} remeshes, {fc.ncells} leaves at the end", flush=True)
+    ctx = Context(wl.cfg)
+    ctx.upload_mesh(wl.mesh)
+    ctx.ts_setup(_params(wl, True))
+    X = torch.as_tensor(H.initial_state(wl)).to(ctx.device)
+    run = TL.solve(ctx, wl.cfg, wl.forest, X, _params(wl, True), times=TIMES, rates=RATES, amrinterval=INTERVAL)
+    torch.cuda.synchronize()
+    print(f"   gpu: {len(run.chunks)} chunks, {run.remeshes} remeshes, {run.forest.ncells} leaves at the end",
+          flush=True)
+    assert run.remeshes == rem_c > 10
+    assert len(run.chunks) == len(chunks_c)
+    for g, c in zip(run.chunks, chunks_c):
+        assert (g[0], g[1], g[2], g[5]) == (c[0], c[1], c[2], c[5]), (g, c)
+        assert g[3] == pytest.approx(c[3], rel=1e-10)
+        assert g[4] == pytest.approx(c[4], rel=1e-8)
+    assert run.t == sum(TIMES)
+    assert np.array_equal(run.forest.level, fc.level) and np.array_equal(run.forest.anchor, fc.anchor)
+    Xg = run.X.cpu().numpy()
+    L = wl.layout
+    m = H.state_mask(Xc, L, wl.cfg)
+    assert np.array_equal(H.state_mask(Xg, L, wl.cfg), m), "active sets differ"
+    errs = H.field_errors(np.where(m, Xg, 0.0), np.where(m, Xc, 0.0), L)
+    print("   final state errors", errs, flush=True)
+    assert errs["PF"] <= 1e-9
+    T0 = H.initial_state(wl).reshape(-1, L["ndof"])[0, L["tm_offset"]]
+    T = Xg.reshape(-1, L["ndof"])[:, L["tm_offset"]]
+    np.testing.assert_allclose(T, T0 + RATES[0] * TIMES[0] + RATES[1] * TIMES[1], rtol=1e-12)
+    ctx.close()
