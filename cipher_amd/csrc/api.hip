@@ -229,7 +229,7 @@ extern "C" int cipher_ctx_create(const cipher_config *cfg, int precision, cipher
         HIPCHK(hipMemcpy(c->tsv, nanv.data(), sizeof(double) * nanv.size(), hipMemcpyHostToDevice));
     }
     if (c->fold) d.tsv = c->tsv;
-    if (hipMalloc(&c->err, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->counts, RHS_COUNTERS * sizeof(int32_t)) != hipSuccess || hipMalloc(&c->partial, sizeof(double) * c->npart) != hipSuccess ||
+    if (hipMalloc(&c->err, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->counts, (RHS_RING + 1) * RHS_COUNTERS * sizeof(int32_t)) != hipSuccess || hipMalloc(&c->partial, sizeof(double) * c->npart) != hipSuccess ||
         hipMalloc(&c->norm, sizeof(double)) != hipSuccess ||
         hipHostMalloc(&c->h_norm, sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&c->h_err, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
@@ -419,7 +419,30 @@ int cipher_rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active,
     L.cm = c->cm;
     L.redo = c->list;
     L.bins = c->bins;
-    L.counts = c->counts;
+    // counters: an eager call takes the next set of the ring (the first call
+    // of a round clears the whole ring); a captured call, or a stage of the
+    // integrator, uses the last set and clears it itself, so a replayed
+    // graph starts from zero every time
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusActive;
+    if (bins_mode == 0 && hipStreamIsCapturing(c->stream, &cap) != hipSuccess) {
+        (void)hipGetLastError();     // unknown: treat as captured
+        cap = hipStreamCaptureStatusActive;
+    }
+    if (bins_mode == 0 && cap == hipStreamCaptureStatusNone) {
+        if (c->ring_pos >= RHS_RING) {
+            L.fill = c->counts;
+            L.nfill = RHS_RING * RHS_COUNTERS;
+            c->ring_pos = 0;
+        } else {
+            L.fill = nullptr;
+            L.nfill = 0;
+        }
+        L.counts = c->counts + (size_t)c->ring_pos++ * RHS_COUNTERS;
+    } else {
+        L.counts = c->counts + (size_t)RHS_RING * RHS_COUNTERS;
+        L.fill = L.counts;
+        L.nfill = RHS_COUNTERS;
+    }
     L.ncells = c->ncells;
     for (int q = 0; q < 5; q++) L.shape[q] = c->shape[q];
     L.bins_mode = bins_mode;

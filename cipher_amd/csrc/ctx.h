@@ -74,7 +74,12 @@ struct cipher_ctx {
     int32_t *glist = nullptr;
     int32_t *list = nullptr;        // FP32-mixed recomputation lists (5 * ncells)
     int32_t *bins = nullptr;        // cells by active-set size (5 * ncells)
-    int32_t *counts = nullptr;      // list and recomputation-list sizes (RHS_COUNTERS)
+    // list and recomputation-list sizes: RHS_RING + 1 sets of RHS_COUNTERS.
+    // Eager cipher_rhs calls take the ring's sets in turn, all of them
+    // cleared by one fill every RHS_RING calls; stream capture and the
+    // integrator's stages use the last set, cleared by each binning call.
+    int32_t *counts = nullptr;
+    int ring_pos = cipher::RHS_RING;
     int32_t *err = nullptr;
     double *partial = nullptr;
     double *norm = nullptr;

@@ -37,6 +37,8 @@ struct ListShape {
 // per-mesh list counters: 0..4 the per-size lists, 8..12 their FP32-mixed
 // recomputation lists
 constexpr int RHS_COUNTERS = 16;
+// counter sets that eager right-hand sides take in turn between clears
+constexpr int RHS_RING = 64;
 
 // cells per binning tile: 256 threads of BIN_CPT consecutive cells
 constexpr int BIN_CPT = 8;
@@ -71,6 +73,8 @@ struct RhsLaunch {
     void *cm;
     int32_t *bins;            // 5 lists of ncells entries (cells by active-set size)
     int32_t *counts;          // RHS_COUNTERS counters
+    int32_t *fill;            // cleared before binning: nfill ints from fill
+    int nfill;
     int32_t *redo;            // 5 recomputation lists of ncells entries
     long long ncells;
     ListShape shape[5];

@@ -1366,7 +1366,8 @@ __global__ __launch_bounds__(256) void k_bin(const double *__restrict__ X, long 
 
 template <int U>
 __global__ void k_fill_i32(int32_t *p, int n, int32_t v) {
-    if ((int)threadIdx.x < n) p[threadIdx.x] = v;
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = v;
 }
 
 
@@ -1415,12 +1416,15 @@ static cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st)
     a.cells = nullptr; a.cells_count = nullptr;
     a.redo = nullptr; a.redo_count = nullptr; a.map = 0; a.pad_ = 0;
     // counts[0..4]: list sizes, counts[8..12]: their FP32-mixed recomputation
-    // lists.  A call that bins clears them all; a call that reuses the lists
-    // of the previous one (same active sets) clears only the recomputation
-    // counters, and only in FP32-mixed.
+    // lists.  A call that bins finds them cleared (L.fill covers them: this
+    // call's own set, or a whole ring of sets cleared once for many calls);
+    // a call that reuses the lists of the previous one (same active sets)
+    // clears only the recomputation counters, and only in FP32-mixed.
     const int nbins = G >= 16 ? 5 : (G >= 8 ? 4 : 3);
     if (L.bins_mode < 2) {
-        hipLaunchKernelGGL(k_fill_i32<0>, dim3(1), dim3(64), 0, st, L.counts, RHS_COUNTERS, 0);
+        if (L.nfill > 0)
+            hipLaunchKernelGGL(k_fill_i32<0>, dim3((unsigned)((L.nfill + 255) / 256)), dim3(256), 0, st, L.fill,
+                               L.nfill, 0);
         hipLaunchKernelGGL(k_bin<0>, dim3((unsigned)((N + BIN_TILE - 1) / BIN_TILE)), dim3(256), 0, st, L.X, N, c,
                            nbins, L.bins, N, L.counts);
     } else if (precision == CIPHER_FP32_MIXED) {
