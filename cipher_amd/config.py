@@ -501,6 +501,8 @@ def load_config(text, loadcase=0):
     cfg.maxnrefine = opt("maxnrefine", 0, _atoi)
     cfg.minnrefine = opt("minnrefine", 0, _atoi)
     cfg.amrinterval = opt("amrinterval", 1 << 30, _atoi)
+    cfg.outputfreq = opt("outputfreq", 1, _atoi)             # src/init.c:1297-1299
+    cfg.outfile = opt("outfile", "output", str)              # src/init.c:1300-1302
     cfg.reltol = opt("reltol", 1.0e-6)
     cfg.abstol = opt("abstol", 1.0e-6)
     cfg.random_seed = opt("random_seed", 0, _atoi)

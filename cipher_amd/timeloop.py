@@ -13,7 +13,10 @@ the mesh is flattened and uploaded, and the integrator is set up afresh (the
 reference destroys and re-creates its TS, src/cipher.c:1826-1827); an adapt
 that changes nothing keeps everything (DMAdaptLabel returns no forest).
 Nucleation sites, which also mark cells for refinement in the reference, are
-out of scope.
+out of scope.  With an output directory, chunks also end at every
+outputfreq-th accepted step, where the output step of PostStep
+(src/cipher.c:976-1091) writes "<outfile>_<step>.vtu" (``vtu.write``) with
+the header's output names computed on the device (cipher_output).
 
 The state moves through host memory at a remesh (the forest and its
 flattening are host data structures, as p4est's are).
@@ -53,17 +56,35 @@ class Run:
         self.mesh = mesh
         self.X = X
         self.chunks = []        # (load case, steps, rejected, t, dt, ncells)
+        self.outputs = []       # VTU files written
         self.remeshes = 0
         self.nsteps = 0
         self.t = 0.0
 
 
-def solve(ctx, cfg, forest, X, params, t0=0.0, times=None, rates=(), amrinterval=None, on_remesh=None):
+def write_output(ctx, cfg, forest, X, step, output_dir):
+    """The output step of PostStep (src/cipher.c:976-1091): the header's
+    output names computed on the device (cipher_output) and written to
+    "<outfile>_<step>.vtu" in output_dir."""
+    import os
+    from . import vtu
+    names = list(cfg.output_names)
+    out = ctx.output(X, names).cpu().numpy() if names else np.zeros((ctx.ncells, 0))
+    path = os.path.join(output_dir, f"{cfg.outfile}_{step}.vtu")
+    vtu.write(path, forest, {nm: out[:, i] for i, nm in enumerate(names)})
+    return path
+
+
+def solve(ctx, cfg, forest, X, params, t0=0.0, times=None, rates=(), amrinterval=None, on_remesh=None,
+          output_dir=None):
     """The loop of src/cipher.c:1589-1606 with the remesh of :1607-1830 on a
     Context whose mesh is forest's flattening and whose integrator is set up
     with params (params.dt_min is timestepmin).  X (device tensor) is
     replaced at each remesh; returns a Run holding the final forest, mesh
-    and state.  times / rates / amrinterval default to the configuration's."""
+    and state.  times / rates / amrinterval default to the configuration's.
+    With output_dir, chunks also end at every outputfreq-th step, where the
+    output step writes its VTU file (before a remesh at the same step, as
+    PostStep runs inside the solve); Run.outputs lists the files."""
     times = list(cfg.times if times is None else times)
     if rates == ():
         rates = cfg.temperature_rates
@@ -72,6 +93,9 @@ def solve(ctx, cfg, forest, X, params, t0=0.0, times=None, rates=(), amrinterval
     interval = int(cfg.amrinterval if amrinterval is None else amrinterval)
     if interval < 1:
         raise ValueError("amrinterval must be >= 1")
+    ofreq = int(cfg.outputfreq) if output_dir is not None else 0
+    if output_dir is not None and ofreq < 1:
+        raise ValueError("outputfreq must be >= 1")
     torch = ctx.torch
     run = Run(forest, ctx.mesh, X)
     current = loadcase_time = t0
@@ -83,12 +107,17 @@ def solve(ctx, cfg, forest, X, params, t0=0.0, times=None, rates=(), amrinterval
         dt = params.dt_min
         end = loadcase_time + tau
         while current < end:
-            st, dt = ctx.ts_solve(run.X, current, end, dt, max_steps=interval - nsteps % interval)
+            chunk = interval - nsteps % interval
+            if ofreq:
+                chunk = min(chunk, ofreq - nsteps % ofreq)
+            st, dt = ctx.ts_solve(run.X, current, end, dt, max_steps=chunk)
             if st.steps == 0 and st.t == current:
                 raise RuntimeError(f"integrator made no progress at t={current}")
             current = st.t
             nsteps += st.steps
             run.chunks.append((k, st.steps, st.rejected, st.t, dt, ctx.ncells))
+            if ofreq and nsteps % ofreq == 0:
+                run.outputs.append(write_output(ctx, cfg, run.forest, run.X, nsteps, output_dir))
             if nsteps % interval == 0:
                 torch.cuda.synchronize()
                 Xh = run.X.cpu().numpy()

@@ -351,20 +351,22 @@ def cpu_loadcases(wl, X, times, rates, params, t0=0.0):
     return X, out
 
 
-def cpu_timeloop(cfg, forest, X, params, times, rates, interval, t0=0.0):
+def cpu_timeloop(cfg, forest, X, params, times, rates, interval, t0=0.0, ofreq=0):
     """CIPHER's time loop with remeshing (src/cipher.c:1589-1830) over
     cpu_solve, in the arithmetic of cipher_amd.timeloop.solve: chunks end at
     multiples of interval accepted steps or at the case's end; after a chunk
     ending on a multiple the host remesh (timeloop.adapt_state) runs and, if
     the forest changed, the controller memory starts afresh (a new TS).
-    Returns (forest, X, chunks, remeshes)."""
+    With ofreq, chunks also end at multiples of ofreq steps, where the
+    state is recorded.  Returns (forest, X, chunks, remeshes) and, with
+    ofreq, the list of (step, forest, X) at the output steps."""
     import types
     from cipher_amd import abi
     from cipher_amd import timeloop as TL
     s = cfg.struct
     saved = (s.has_temperature_rate, s.temperature_rate)
     wl = types.SimpleNamespace(cfg=cfg, mesh=forest.flatten())
-    ctrl, chunks, remeshes = abi.TsCtrl(), [], 0
+    ctrl, chunks, remeshes, outputs = abi.TsCtrl(), [], 0, []
     current = case_t = t0
     nsteps = 0
     X = np.array(X, dtype=np.float64)
@@ -375,12 +377,16 @@ def cpu_timeloop(cfg, forest, X, params, times, rates, interval, t0=0.0):
             dt = params.dt_min
             end = case_t + tau
             while current < end:
-                X, info = cpu_solve(wl, X, current, end, dt, params, max_steps=interval - nsteps % interval,
-                                    ctrl=ctrl)
+                chunk = interval - nsteps % interval
+                if ofreq:
+                    chunk = min(chunk, ofreq - nsteps % ofreq)
+                X, info = cpu_solve(wl, X, current, end, dt, params, max_steps=chunk, ctrl=ctrl)
                 assert info["steps"] > 0
                 current, dt = info["t"], info["dt"]
                 nsteps += info["steps"]
                 chunks.append((k, info["steps"], info["rejected"], current, dt, wl.mesh.ncells))
+                if ofreq and nsteps % ofreq == 0:
+                    outputs.append((nsteps, forest, X.copy()))
                 if nsteps % interval == 0:
                     f2, X2 = TL.adapt_state(forest, X, cfg.layout(), cfg.minnrefine, cfg.maxnrefine)
                     if f2 is not forest:
@@ -391,6 +397,8 @@ def cpu_timeloop(cfg, forest, X, params, times, rates, interval, t0=0.0):
             case_t += tau
     finally:
         s.has_temperature_rate, s.temperature_rate = saved
+    if ofreq:
+        return forest, X, chunks, remeshes, outputs
     return forest, X, chunks, remeshes
 
 

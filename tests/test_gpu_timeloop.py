@@ -8,7 +8,9 @@ leaves): every chunk's case, accepted and rejected counts and leaf count
 equal, times and next step sizes within 1e-10, the same final forest, the
 same active sets and phase fractions within 1e-9, and the temperature
 T0 + sum_k rate_k * time_k in every leaf (refinement copies it, coarsening
-averages equal values).
+averages equal values).  With outputfreq 10 the output step writes a VTU
+file at every 10th step: each file's leaves and phase ids equal the oracle's
+output fields (src/cipher.c:1005-1080) of the CPU run's state at that step.
 """
 import numpy as np
 import pytest
@@ -27,20 +29,28 @@ def _params(wl, graph):
     return abi.ts_params(1e-4, 1e-4, wl.cfg.timestepmin, 100.0, use_graph=graph)
 
 
-def test_timeloop_with_remeshing_matches_cpu():
+def test_timeloop_with_remeshing_matches_cpu(tmp_path):
     import torch
+    import oracle
     from cipher_amd.runtime import Context
     from cipher_amd import timeloop as TL
+    from cipher_amd import vtu
     wl = H.gg2d_amr(amrinterval=INTERVAL)
     print(f"[timeloop] {wl.ncells} leaves, cases {TIMES} rates {RATES}, remesh every {INTERVAL} steps", flush=True)
-    fc, Xc, chunks_c, rem_c = H.cpu_timeloop(wl.cfg, wl.forest, H.initial_state(wl), _params(wl, False),
-                                             TIMES, RATES, INTERVAL)
+    fc, Xc, chunks_c, rem_c, outs_c = H.cpu_timeloop(wl.cfg, wl.forest, H.initial_state(wl), _params(wl, False),
+                                                     TIMES, RATES, INTERVAL, ofreq=INTERVAL)
     print(f"   cpu: {len(chunks_c)} chunks, {rem_c} remeshes, {fc.ncells} leaves at the end", flush=True)
     ctx = Context(wl.cfg)
     ctx.upload_mesh(wl.mesh)
     ctx.ts_setup(_params(wl, True))
     X = torch.as_tensor(H.initial_state(wl)).to(ctx.device)
-    run = TL.solve(ctx, wl.cfg, wl.forest, X, _params(wl, True), times=TIMES, rates=RATES, amrinterval=INTERVAL)
+    saved = wl.cfg.outputfreq
+    wl.cfg.outputfreq = INTERVAL
+    try:
+        run = TL.solve(ctx, wl.cfg, wl.forest, X, _params(wl, True), times=TIMES, rates=RATES,
+                       amrinterval=INTERVAL, output_dir=str(tmp_path))
+    finally:
+        wl.cfg.outputfreq = saved
     torch.cuda.synchronize()
     print(f"   gpu: {len(run.chunks)} chunks, {run.remeshes} remeshes, {run.forest.ncells} leaves at the end",
           flush=True)
@@ -62,4 +72,18 @@ def test_timeloop_with_remeshing_matches_cpu():
     T0 = H.initial_state(wl).reshape(-1, L["ndof"])[0, L["tm_offset"]]
     T = Xg.reshape(-1, L["ndof"])[:, L["tm_offset"]]
     np.testing.assert_allclose(T, T0 + RATES[0] * TIMES[0] + RATES[1] * TIMES[1], rtol=1e-12)
+    # output steps
+    assert [p.rsplit("_", 1)[1] for p in run.outputs] == [f"{st}.vtu" for st, _, _ in outs_c]
+    D = L["ndof"]
+    for path, (st, f, Xs) in zip(run.outputs, outs_c):
+        _, conn, _, data = vtu.read(path)
+        want = oracle.output(wl.cfg, f.flatten(), Xs, [(abi.OUT_PHASEID, 0)])[:, 0]
+        assert conn.shape[0] == f.ncells, path
+        # equal ids, except where the two largest interpolants of the CPU
+        # state tie within the states' 1e-9 agreement
+        for c in np.flatnonzero(data["phaseid"] != want):
+            n = int(round(Xs[c * D]))
+            h = np.sort(oracle.interpolant(wl.cfg.interpolation, Xs[c * D + L["pf_offset"]:c * D + L["pf_offset"] + n]))
+            assert n > 1 and h[-1] - h[-2] < 1e-9, (path, c)
+    print(f"   {len(run.outputs)} output files checked", flush=True)
     ctx.close()
