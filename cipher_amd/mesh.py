@@ -326,7 +326,7 @@ def adapt(forest, refine, minlevel, maxlevel):
     ok = np.ones(uk.size, dtype=bool)
     np.logical_and.at(ok, inv, coarsen)
     merge = ok[inv] & (cnt[inv] == 1 << dim)
-    split = refine & (f.level < maxlevel)
+    split = refine & (f.level < min(maxlevel, f.lmax))
     keep = ~merge & ~split
     lv = [f.level[keep]]
     an = [f.anchor[keep]]
