@@ -7,9 +7,9 @@ PETSc/p4est are not in the image, so the adapt is pinned by its invariants:
 leaf labelled for refinement below the maximum level is split, no leaf ends
 more than one level coarser than before, a coarsened leaf is exactly one
 family of unlabelled leaves above the minimum level, uniform forests refine
-and coarsen to uniform forests and stop at the level limits; the transfer copies records into refined leaves, averages
-them into coarsened ones, and so conserves the volume integral of every
-field.
+and coarsen to uniform forests and stop at the level limits; the transfer
+copies records into refined leaves, averages them into coarsened ones, and
+so conserves the volume integral of every field.
 """
 import numpy as np
 import pytest
