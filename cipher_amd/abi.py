@@ -170,6 +170,7 @@ def _declare(lib):
         "cipher_batch_offsets": (i32, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "cipher_batch_set_precision": (i32, [vp, i32]),
         "cipher_batch_set_stream": (i32, [vp, vp]),
+        "cipher_batch_set_temperature_rate": (i32, [vp, i32, i32, dbl]),
         "cipher_batch_rhs": (i32, [vp, dbl, vp, vp]),
         "cipher_batch_rhs_active": (i32, [vp, dbl, vp, vp, vp]),
         "cipher_batch_poststep": (i32, [vp, vp, C.POINTER(C.c_int32)]),
@@ -199,7 +200,8 @@ EXPORTED = ("cipher_layout_of", "cipher_ctx_create", "cipher_ctx_destroy", "ciph
             "cipher_ts_adapt", "cipher_ts_setup", "cipher_ts_attempt", "cipher_ts_accept", "cipher_ts_solve",
             "cipher_ts_loadcases",
             "cipher_batch_create", "cipher_batch_destroy", "cipher_batch_offsets", "cipher_batch_set_precision",
-            "cipher_batch_set_stream", "cipher_batch_rhs", "cipher_batch_rhs_active", "cipher_batch_poststep",
+            "cipher_batch_set_stream", "cipher_batch_set_temperature_rate", "cipher_batch_rhs",
+            "cipher_batch_rhs_active", "cipher_batch_poststep",
             "cipher_batch_wrms_norm", "cipher_batch_autotune", "cipher_batch_shape", "cipher_batch_ts_setup",
             "cipher_batch_ts_solve", "cipher_get_shape", "cipher_set_shape", "cipher_autotune", "cipher_output")
 

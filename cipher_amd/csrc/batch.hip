@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -61,6 +62,9 @@ struct BatchTs {
     hipGraphExec_t exec = nullptr;
     const double *graph_X = nullptr;
     hipStream_t own = nullptr;
+    // per-member step-controller memory, carried from call to call as a
+    // context's is (cleared by cipher_batch_ts_setup)
+    std::vector<cipher_ts_ctrl> ctrl;
 };
 
 }  // namespace
@@ -412,6 +416,24 @@ extern "C" int cipher_batch_set_stream(cipher_batch *b, void *stream) {
     return CIPHER_OK;
 }
 
+extern "C" int cipher_batch_set_temperature_rate(cipher_batch *b, int32_t member, int has_rate, double rate) {
+    if (!b) return cipher_fail(CIPHER_ERR_ARG, "null batch");
+    if (member < 0 || member >= (int32_t)b->m.size()) return cipher_fail(CIPHER_ERR_ARG, "member out of range");
+    cipher_ctx *c = b->m[member];
+    int rc = cipher_set_temperature_rate(c, has_rate, rate);
+    if (rc) return rc;
+    // the two fields, from the context's updated configuration into the
+    // member's copy on the device, ordered after the batch's earlier work
+    const size_t lo = offsetof(DevCfg, trate), hi = offsetof(DevCfg, has_trate) + sizeof(int);
+    char *dst = (char *)(b->d_mem + member) + offsetof(BatchMember, c) + lo;
+    const hipStream_t st = b->stream;
+    if (hipMemcpyAsync(dst, (const char *)&c->dc + lo, hi - lo, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return cipher_fail(CIPHER_ERR_HIP, "member table update");
+    ts_invalidate(b);
+    return CIPHER_OK;
+}
+
 // zero the recomputation counters (8..15) of every member, keeping the lists
 __global__ void k_fill_redo(int32_t *counts, int nmem) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -605,6 +627,7 @@ extern "C" int cipher_batch_ts_setup(cipher_batch *b, const cipher_ts_params *p)
     BCHK(hipMalloc(&s.Xn, bytes));
     BCHK(hipMalloc(&s.Xh, bytes));
     BCHK(hipMalloc(&s.dt_dev, sizeof(double) * b->m.size()));
+    s.ctrl.assign(b->m.size(), cipher_ts_ctrl{});
     s.ready = true;
     return CIPHER_OK;
 }
@@ -668,7 +691,7 @@ extern "C" int cipher_batch_ts_solve(cipher_batch *b, double *X, const double *t
         restore = true;
     }
     std::vector<double> t(t0, t0 + n), dt(n), dt_try(n);
-    std::vector<cipher_ts_ctrl> ctrl(n);
+    std::vector<cipher_ts_ctrl> &ctrl = s.ctrl;
     std::vector<cipher_ts_stats> out(n);
     for (int i = 0; i < n; i++) dt[i] = dt_inout[i] > 0.0 ? dt_inout[i] : s.p.dt_min;
     int rc = CIPHER_OK;

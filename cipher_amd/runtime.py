@@ -309,6 +309,7 @@ class Batch:
         return out
 
     def ts_setup(self, params):
+        self.ts_params = params
         abi.check(self.lib.cipher_batch_ts_setup(self.h, C.byref(params)), "cipher_batch_ts_setup")
 
     def ts_solve(self, X, t0, t_end, dt, max_steps=0):
@@ -323,6 +324,42 @@ class Batch:
         st = (abi.TsStats * n)()
         abi.check(self.lib.cipher_batch_ts_solve(self.h, _ptr(X), a0, a1, d, max_steps, st), "cipher_batch_ts_solve")
         return list(st), list(d)
+
+    def set_temperature_rate(self, member, rate):
+        """Member's load-case temperature rate (None: no rate)."""
+        abi.check(self.lib.cipher_batch_set_temperature_rate(self.h, member, 0 if rate is None else 1,
+                                                             0.0 if rate is None else float(rate)),
+                  "cipher_batch_set_temperature_rate")
+
+    def solve_loadcases(self, X, times, rates=None, t0=0.0):
+        """Every member through its load cases at once, each as
+        cipher_ts_loadcases does on its own context (src/cipher.c:1589-1606):
+        times[i][k] and rates[i][k] (rates None: no temperature rate) are
+        member i's case lengths and rates, all members with the same number
+        of cases; case k restarts every member at the integrator's dt_min
+        and ends it at its running sum of case lengths.  Returns, per case,
+        the list of per-member TsStats."""
+        n = len(self.members)
+        if len(times) != n or (rates is not None and len(rates) != n):
+            raise ValueError("one list of case lengths (and rates) per member")
+        ncase = len(times[0])
+        if any(len(tm) != ncase for tm in times) or (rates is not None and any(len(r) != ncase for r in rates)):
+            raise ValueError("every member needs the same number of load cases")
+        if any(tau < 0.0 for tm in times for tau in tm):
+            raise ValueError("load-case time must be >= 0")
+        dt_min = self.ts_params.dt_min
+        start = [t0] * n
+        cur = [t0] * n
+        out = []
+        for k in range(ncase):
+            for i in range(n):
+                self.set_temperature_rate(i, None if rates is None else rates[i][k])
+            end = [start[i] + times[i][k] for i in range(n)]
+            stats, _ = self.ts_solve(X, cur, end, [dt_min] * n)
+            cur = [st.t if cur[i] < end[i] else cur[i] for i, st in enumerate(stats)]
+            out.append(stats)
+            start = end
+        return out
 
     def close(self):
         if getattr(self, "h", None):

@@ -397,6 +397,10 @@ int cipher_batch_destroy(cipher_batch *b);
 int cipher_batch_offsets(const cipher_batch *b, int64_t *offsets, int64_t *cell_offsets);
 int cipher_batch_set_precision(cipher_batch *b, int precision);
 int cipher_batch_set_stream(cipher_batch *b, void *stream);
+/* Member i's load-case temperature rate (cipher_set_temperature_rate on its
+ * context, src/cipher.c:568, and in the batch's member table), for load
+ * cases run through the batch.  Synchronises the batch's stream. */
+int cipher_batch_set_temperature_rate(cipher_batch *b, int32_t member, int has_rate, double rate);
 
 /* Every member's cipher_rhs (cipher_rhs_active) in one batched launch set;
  * active_dev is indexed by batch cell (cell_offsets). */
@@ -428,7 +432,9 @@ int cipher_batch_shape(const cipher_batch *b, int32_t k, int32_t q, int32_t *ncp
  * each with its own step size, controller and end time; an attempt advances
  * all members that have not reached their end time (or max_steps accepted
  * steps) and is replayed from one captured HIP graph.  t0, t_end, dt_inout
- * and stats have nmembers entries. */
+ * and stats have nmembers entries.  Each member's controller memory carries
+ * over from call to call, as a context's does; cipher_batch_ts_setup clears
+ * it. */
 int cipher_batch_ts_setup(cipher_batch *b, const cipher_ts_params *p);
 int cipher_batch_ts_solve(cipher_batch *b, double *X_dev, const double *t0, const double *t_end,
                           double *dt_inout, int64_t max_steps, cipher_ts_stats *stats);

@@ -193,3 +193,41 @@ def test_batch_autotune_keeps_results():
         assert (1 << q) <= lanes <= capl and mp in (0, 1) and cap >= 1
     assert np.array_equal(_host(b.rhs(X)), F0)
     b.close()
+
+
+def test_batch_loadcases_equal_members_alone():
+    """Load cases through the batch (per-member case lengths and temperature
+    rates, cipher_batch_set_temperature_rate) give each member the counts,
+    times and state of cipher_ts_loadcases on its own context, bit for bit."""
+    import torch
+    from cipher_amd.runtime import Batch
+    wls = H.ensemble()
+    states = H.ensemble_states()
+    times, rates = [], []
+    for i, wl in enumerate(wls):
+        scale = 1.0 if wl.layout["dp_size"] == 0 else 1e-9
+        times.append([scale * 0.3, 0.0, scale * (0.2 + 0.1 * i)])
+        rates.append([5.0e3 * (i + 1), 1.0e9, -2.0e4])
+    params = abi.ts_params(1e-4, 1e-4, 1e-16, 1.0, use_graph=True)
+    ctxs = _contexts(wls)
+    ref_X, ref_st = [], []
+    for i, (c, X) in enumerate(zip(ctxs, states)):
+        c.ts_setup(params)
+        Xd = torch.as_tensor(X).to(c.device)
+        st = c.solve_loadcases(Xd, times[i], rates[i])
+        ref_X.append(_host(Xd))
+        ref_st.append([(s.steps, s.rejected, s.t, s.dt) for s in st])
+    b = Batch(ctxs)
+    b.ts_setup(params)
+    X = _cat(states)
+    print(f"[batch-loadcases] {len(wls)} members, 3 cases each", flush=True)
+    per_case = b.solve_loadcases(X, times, rates)
+    got = _host(X)
+    for i, wl in enumerate(wls):
+        mine = [(cs[i].steps, cs[i].rejected, cs[i].t, cs[i].dt) for cs in per_case]
+        print(f"   {wl.name}: {mine}", flush=True)
+        assert [m[:3] for m in mine] == [r[:3] for r in ref_st[i]], f"member {i} ({wl.name})"
+        # the next step size of a skipped (zero-length) case is dt_min in both
+        assert [m[3] for m in mine if m[0]] == [r[3] for r in ref_st[i] if r[0]]
+        assert np.array_equal(got[b.offsets[i]:b.offsets[i + 1]], ref_X[i]), f"member {i} ({wl.name}): state differs"
+    b.close()
