@@ -347,6 +347,8 @@ class Batch:
             raise ValueError("every member needs the same number of load cases")
         if any(tau < 0.0 for tm in times for tau in tm):
             raise ValueError("load-case time must be >= 0")
+        if getattr(self, "ts_params", None) is None:
+            raise RuntimeError("Batch.ts_setup must be called before solve_loadcases")
         dt_min = self.ts_params.dt_min
         start = [t0] * n
         cur = [t0] * n
