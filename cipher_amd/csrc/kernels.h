@@ -39,12 +39,6 @@ struct ListShape {
 constexpr int RHS_COUNTERS = 16;
 // counter sets that eager right-hand sides take in turn between clears
 constexpr int RHS_RING = 64;
-// most blocks of a list in a context's FP32-mixed FP64 recomputation launch
-// (64 against the first pass's caps: 76 -> 73 us per FP32-mixed 3D
-// grain-growth RHS, 705 -> 695 us on the ternary one).  A batch keeps the
-// first pass's segments: shares of 64 blocks leave a member with a long
-// recomputation list one block (10.4 -> 14.1 ms on the 96-member ensemble).
-constexpr int RHS_REDO_CAP = 64;
 
 // cells per binning tile: 256 threads of BIN_CPT consecutive cells
 constexpr int BIN_CPT = 8;

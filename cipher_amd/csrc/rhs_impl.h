@@ -1450,20 +1450,10 @@ static cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st)
     if (precision == CIPHER_FP32_MIXED) {
         hipLaunchKernelGGL((k_rhs_all<G, NC, float>), dim3((unsigned)tot), dim3(256), lds, st, a, c, L.bins, L.redo,
                            L.counts, rg);
-        // FP64 recomputation of the cells whose flags sat in the FP32 band:
-        // those lists are short, so each gets at most RHS_REDO_CAP blocks,
-        // which stride over whatever it holds
-        ListShape rs[5];
-        for (int q = 0; q < 5; q++) {
-            rs[q] = L.shape[q];
-            rs[q].cap = std::min(rs[q].cap, RHS_REDO_CAP);
-        }
-        size_t lr = 0;
-        long long tr = 0;
-        ListRanges rr = make_ranges(rs, N, c, nbins, &lr, &tr);
-        rr.redo = 1;
-        hipLaunchKernelGGL((k_rhs_all<G, NC, double>), dim3((unsigned)tr), dim3(256), lr, st, a, c, L.bins, L.redo,
-                           L.counts, rr);
+        // FP64 recomputation of the cells whose flags sat in the FP32 band
+        rg.redo = 1;
+        hipLaunchKernelGGL((k_rhs_all<G, NC, double>), dim3((unsigned)tot), dim3(256), lds, st, a, c, L.bins, L.redo,
+                           L.counts, rg);
     } else {
         hipLaunchKernelGGL((k_rhs_all<G, NC, double>), dim3((unsigned)tot), dim3(256), lds, st, a, c, L.bins, L.redo,
                            L.counts, rg);
