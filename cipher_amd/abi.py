@@ -24,6 +24,7 @@ STATUS = {0: "CIPHER_OK", 1: "CIPHER_ERR_ARG", 2: "CIPHER_ERR_HIP",
           5: "CIPHER_ERR_UNSUPPORTED", 6: "CIPHER_ERR_STATE"}
 ERR_CAPACITY = 3
 ERR_ARG = 1
+ERR_UNSUPPORTED = 5
 
 
 class TSeries(C.Structure):

@@ -512,6 +512,10 @@ def load_config(text, loadcase=0):
     cfg.interpolation = interp_id
     cfg.materials, cfg.interfaces, cfg.bcs = mats, ifaces, bcs
     cfg.periodic = [False] * dim
+    # TSRK scheme of the time loop (src/cipher.c:1104 sets 3bs; a YAML
+    # petscoptions '-ts_rk_type <t>' overrides it through TSSetFromOptions)
+    mo = re.search(r"-ts_rk_type\s+(\S+)", cfg.petscoptions)
+    cfg.ts_rk_type = mo.group(1) if mo else "3bs"
     mo = re.search(r"-dm_p4est_brick_periodicity\s+([0-9,]+)", cfg.petscoptions)
     if mo:
         flags = [_atoi(v) for v in mo.group(1).split(",") if v]

@@ -85,6 +85,12 @@ def solve(ctx, cfg, forest, X, params, t0=0.0, times=None, rates=(), amrinterval
     With output_dir, chunks also end at every outputfreq-th step, where the
     output step writes its VTU file (before a remesh at the same step, as
     PostStep runs inside the solve); Run.outputs lists the files."""
+    if getattr(cfg, "ts_rk_type", "3bs") != "3bs":
+        # the device integrator is Bogacki-Shampine 3(2); integrating a
+        # configuration written for another scheme would change its
+        # trajectory and step counts without notice
+        raise abi.CipherError(abi.ERR_UNSUPPORTED, f"petscoptions -ts_rk_type {cfg.ts_rk_type} "
+                              "(only 3bs is integrated)")
     times = list(cfg.times if times is None else times)
     if rates == ():
         rates = cfg.temperature_rates

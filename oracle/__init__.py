@@ -12,7 +12,9 @@ import numpy as np
 from cipher_amd import abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(_HERE, "liboracle.so")
+# CIPHER_ORACLE_LIB: a restatement built elsewhere (oracle/mutate.py points
+# the pins at planted-error builds through it)
+LIB = os.environ.get("CIPHER_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 

@@ -161,7 +161,7 @@ def test_attempt_matches_cpu(name):
     Xd = _dev(ctx, X)
     err_g = ctx.ts_attempt(Xd, dt)
     Xn, _, err_c = H.cpu_attempt(wl, X, dt, p.atol, p.rtol)
-    assert err_g == pytest.approx(err_c, rel=1e-8)
+    assert err_g == pytest.approx(err_c, rel=1e-8, abs=0.0)
     ctx.ts_accept(Xd)
     Xo, rc = oracle.poststep(wl.cfg, wl.mesh, Xn)
     m = H.state_mask(Xo, L, wl.cfg)
@@ -190,7 +190,7 @@ def test_trajectory_matches_cpu(name):
     print(f"   gpu steps={st.steps} rejected={st.rejected} t={st.t} dt={st.dt}", flush=True)
     assert (st.steps, st.rejected) == (info["steps"], info["rejected"])
     assert st.t == info["t"]
-    assert dt_next == pytest.approx(info["dt"], rel=1e-8)
+    assert dt_next == pytest.approx(info["dt"], rel=1e-8, abs=0.0)
     Xg = _host(Xd)
     L = wl.layout
     m = H.state_mask(Xc, L, wl.cfg)

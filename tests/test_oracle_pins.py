@@ -19,7 +19,7 @@ from cipher_amd import abi
 from cipher_amd import workload as W
 from tests import helpers as H
 
-RG = 8.314
+RG = abi.R_GAS_CONST
 PI = math.pi
 
 
@@ -166,7 +166,7 @@ def test_wrms_norm_closed_form():
     atol, rtol = 1e-4, 1e-3
     tol = atol + rtol * np.maximum(np.abs(U), np.abs(Y))
     ref = math.sqrt(np.mean(((U - Y) / tol) ** 2))
-    assert oracle.wrms_norm(U, Y, atol, rtol) == pytest.approx(ref, rel=1e-13)
+    assert oracle.wrms_norm(U, Y, atol, rtol) == pytest.approx(ref, rel=1e-13, abs=0.0)
     assert oracle.wrms_norm(U, U, atol, rtol) == 0.0
     assert oracle.wrms_norm(U[:0], Y[:0], atol, rtol) == 0.0
 
@@ -299,8 +299,12 @@ def test_mobility_component_andersson_agren(chem):
         mat = _materials(cfg)[key]
         m = cfg.materials[mat]
         T = 610.0
-        y = _random_y(rng, cfg, mat)[:nc]
-        mob = oracle.mobility_component(cfg, mat, y, T)
+        # every site of the phase is passed; the mobility reads the first
+        # sublattice's fractions (src/material.c:1023-1046 indexes
+        # sitefrac[0..ncp) for calphad2sl)
+        yall = _random_y(rng, cfg, mat)
+        y = yall[:nc]
+        mob = oracle.mobility_component(cfg, mat, yall, T)
         for k in range(nc):
             mb = cfg.pools.mob[m.mobility + k]
             Q = sum(y[j] * oracle.sum_tseries(T, cfg.pools.ts[mb.unary + j]) for j in range(nc))
@@ -312,7 +316,7 @@ def test_mobility_component_andersson_agren(chem):
                     Q += y[j] * y[i] * np.polynomial.polynomial.polyval(y[j] - y[i], poly) if poly else 0.0
                     r += 1
             ref = m.molarvolume * y[k] * mb.m0 / (RG * T) * math.exp(Q / (RG * T))
-            assert mob[k] == pytest.approx(ref, rel=1e-12)
+            assert mob[k] == pytest.approx(ref, rel=1e-12, abs=0.0)
     none = _materials(cfg)[abi.CHEM_NONE]
     assert not oracle.mobility_component(cfg, none, np.full(nc, 1.0 / nc), 600.0).any()
 

@@ -78,3 +78,20 @@ def test_anisotropy_case_runs():
     assert np.isfinite(F).all()
     r = F.reshape(-1, wl.layout["ndof"])[:, wl.layout["pf_offset"]:wl.layout["pf_offset"] + wl.layout["pf_size"]]
     assert (np.abs(r.sum(axis=1)) <= 1e-12 * (np.abs(r).max(axis=1) + 1e-300)).all()
+
+
+def test_rk_type_other_than_3bs_is_rejected_by_the_time_loop():
+    """Several reference examples ask for '-ts_rk_type 2a' in petscoptions;
+    the time loop integrates Bogacki-Shampine 3(2) only, so it refuses such a
+    configuration instead of changing its trajectory silently."""
+    import os
+    from cipher_amd import config, timeloop
+    from cipher_amd import workload as W
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    cfg = config.load_config(open(os.path.join(gold, "ref_grain_growth.yaml")).read())
+    assert cfg.ts_rk_type == "2a"
+    with pytest.raises(abi.CipherError) as e:
+        timeloop.solve(None, cfg, None, None, None)
+    assert e.value.code == abi.ERR_UNSUPPORTED
+    gen = config.load_config(W.grain_growth_yaml(res=(16, 16), ngrains=4))
+    assert gen.ts_rk_type == "3bs"
