@@ -156,6 +156,7 @@ def _declare(lib):
         "cipher_poststep": (i32, [vp, vp]),
         "cipher_setup_problem": (i32, [vp, vp, vp]),
         "cipher_wrms_norm": (i32, [vp, vp, vp, i64, dbl, dbl, C.POINTER(C.c_double)]),
+        "cipher_wrms_norm_noalloc": (i32, [vp, vp, i64, dbl, dbl, vp, C.POINTER(C.c_double)]),
         "cipher_axpy_n": (i32, [vp, i64, vp, vp, i32, C.POINTER(C.c_double), C.POINTER(vp)]),
         "cipher_last_error": (C.c_char_p, []),
         "cipher_ts_adapt": (i32, [C.POINTER(TsParams), C.POINTER(TsCtrl), dbl, dbl, C.POINTER(C.c_int),
@@ -197,7 +198,7 @@ def _declare(lib):
 EXPORTED = ("cipher_layout_of", "cipher_ctx_create", "cipher_ctx_destroy", "cipher_ctx_layout",
             "cipher_set_precision", "cipher_set_stream", "cipher_set_temperature_rate",
             "cipher_upload_mesh", "cipher_rhs", "cipher_rhs_active", "cipher_poststep",
-            "cipher_setup_problem", "cipher_wrms_norm", "cipher_axpy_n", "cipher_last_error",
+            "cipher_setup_problem", "cipher_wrms_norm", "cipher_wrms_norm_noalloc", "cipher_axpy_n", "cipher_last_error",
             "cipher_ts_adapt", "cipher_ts_setup", "cipher_ts_attempt", "cipher_ts_accept", "cipher_ts_solve",
             "cipher_ts_loadcases",
             "cipher_batch_create", "cipher_batch_destroy", "cipher_batch_offsets", "cipher_batch_set_precision",

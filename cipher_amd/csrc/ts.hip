@@ -7,7 +7,13 @@
 //   shuffles, fixed-order final sum) so repeated calls give identical bits.
 // k_axpy_n: Y = X + (((a0 K0 + a1 K1) + a2 K2) + ...), the stage/update
 //   combination of an explicit RK step, streamed once through HBM.
+#include <cmath>
+#include <mutex>
+#include <string>
+
 #include "kernels.h"
+
+int cipher_fail(int code, const std::string &msg);
 
 namespace cipher {
 
@@ -267,4 +273,58 @@ cipher_status launch_axpy_dt(long long n, double *Y, const double *X, const doub
     return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
 }
 
+// ---------------------------------------------------------------------------
+// cipher_wrms_norm_noalloc: the weighted RMS norm with its partial sums and
+// result in a file-scope device array instead of a context's buffers, for
+// callers that cannot allocate.  wrms_parts() never exceeds WRMS_MAX_PARTS,
+// so the array covers every n; every call overwrites the partials it reads
+// (nothing needs clearing).  Calls that overlap on one GPU share the array
+// and race.
+constexpr int WRMS_MAX_PARTS = 1024;
+constexpr int WRMS_SCRATCH = WRMS_MAX_PARTS + 32;  // partials, then the norm
+static __device__ __align__(256) double scratch[WRMS_SCRATCH];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch[MAX_DEVICES];
+
+// the array's device address on the current device (looked up once per
+// device; each GPU holds its own copy)
+static int scratch_address(double **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return cipher_fail(CIPHER_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= MAX_DEVICES) return cipher_fail(CIPHER_ERR_UNSUPPORTED, "device index beyond 64");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return cipher_fail(CIPHER_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch[dev];
+    return CIPHER_OK;
+}
+
 }  // namespace cipher
+
+extern "C" int cipher_wrms_norm_noalloc(const double *U, const double *Y, int64_t n, double atol, double rtol,
+                                        void *stream, double *norm_out) {
+    using namespace cipher;
+    if (!U || !Y || !norm_out || n < 0) return cipher_fail(CIPHER_ERR_ARG, "bad argument");
+    const int npart = wrms_parts(n);
+    if (npart > WRMS_MAX_PARTS)
+        return cipher_fail(CIPHER_ERR_ARG, "error norm needs more partial sums than the static scratch holds");
+    double *ptr = nullptr;
+    int rc = scratch_address(&ptr);
+    if (rc != CIPHER_OK) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    cipher_status s = launch_wrms(U, Y, n, atol, rtol, ptr, npart, ptr + WRMS_MAX_PARTS, st);
+    if (s != CIPHER_OK) return cipher_fail(s, "wrms launch");
+    hipError_t e = hipMemcpyAsync(norm_out, ptr + WRMS_MAX_PARTS, sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return cipher_fail(CIPHER_ERR_HIP, std::string("norm copy: ") + hipGetErrorString(e));
+    if (!std::isfinite(*norm_out)) return cipher_fail(CIPHER_ERR_NONFINITE, "error norm is not finite");
+    return CIPHER_OK;
+}

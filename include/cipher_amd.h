@@ -254,6 +254,17 @@ int cipher_setup_problem(cipher_ctx *ctx, const int32_t *phase_dev, double *X_de
 int cipher_wrms_norm(cipher_ctx *ctx, const double *U_dev, const double *Y_dev,
                      int64_t n, double atol, double rtol, double *norm_out);
 
+/* No counterpart in the reference.  cipher_wrms_norm without a context, for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): same kernels, partition and
+ * result bits, with the partial sums in a static device array of the library
+ * (1 056 doubles per GPU, held until the process ends; the fixed partition
+ * uses at most 1 024 partial sums, so every n fits).  Allocates nothing.
+ * stream: a hipStream_t as void* (NULL: the default stream); synchronises
+ * it.  Calls that overlap on one GPU share the array and race. */
+int cipher_wrms_norm_noalloc(const double *U_dev, const double *Y_dev, int64_t n, double atol,
+                             double rtol, void *stream, double *norm_out);
+
 /* Y = X + sum_j a[j]*K_j (j < nk), the RK stage / update combination, FP64.
  * K_dev points to nk device vectors of n doubles each. */
 int cipher_axpy_n(cipher_ctx *ctx, int64_t n, double *Y_dev, const double *X_dev,
