@@ -128,6 +128,23 @@ static int validate(const cipher_config *cfg) {
         if (mt.molarvolume == 0.0) return fail(CIPHER_ERR_ARG, "molarvolume must be nonzero");
         if (mt.model != CIPHER_CHEM_NONE && (mt.mobility < 0 || mt.mobility + cfg->ncp > cfg->nmobility))
             return fail(CIPHER_ERR_ARG, "material mobility index");
+        // Redlich-Kister blocks the constitutive functions walk
+        // (src/material.c:356-451, 495-619, 1023-1074)
+        const int nc = cfg->ncp, p2 = nc * (nc - 1) / 2, p3 = nc * (nc - 1) * (nc - 2) / 6;
+        auto rk_ok = [&](int off, int cnt) { return cnt == 0 || (off >= 0 && off + cnt <= cfg->nrk); };
+        bool ok = true;
+        if (mt.model == CIPHER_CHEM_CALPHADDIS) ok = rk_ok(mt.binary, p2) && rk_ok(mt.ternary, p3);
+        if (mt.model == CIPHER_CHEM_CALPHAD2SL)
+            ok = rk_ok(mt.binary, nc * p2) && rk_ok(mt.binaryq, nc * p2) && rk_ok(mt.ternary, nc * p3) &&
+                 rk_ok(mt.ternaryq, nc * p3);
+        if (mt.model != CIPHER_CHEM_NONE)
+            for (int k = 0; k < nc && ok; k++) ok = rk_ok(cfg->mobility[mt.mobility + k].binary, p2);
+        if (mt.model == CIPHER_CHEM_CALPHADDIS && ok)
+            for (int t = 0; t < p3; t++) {
+                const cipher_rk &rt = cfg->rk[mt.ternary + t];
+                if (rt.n > 0 && (!cfg->rkindex || rt.idx < 0 || rt.idx + rt.n > cfg->nrkindex)) ok = false;
+            }
+        if (!ok) return fail(CIPHER_ERR_ARG, "material Redlich-Kister block out of range");
     }
     for (int i = 0; i < cfg->nf; i++) {
         const cipher_interface &f = cfg->interface[i];
@@ -156,6 +173,13 @@ extern "C" int cipher_ctx_create(const cipher_config *cfg, int precision, cipher
     int r = 0;
     r |= upload(c, cfg->tseries, cfg->ntseries, &d.ts);
     r |= upload(c, cfg->rk, cfg->nrk, &d.rk);
+    {
+        // bit r: rk[r] has terms; two zero words of padding for rk_window
+        std::vector<uint64_t> bits((size_t)(cfg->nrk + 63) / 64 + 2, 0ull);
+        for (int i = 0; i < cfg->nrk; i++)
+            if (cfg->rk[i].n > 0) bits[(size_t)i >> 6] |= 1ull << (i & 63);
+        r |= upload(c, bits.data(), (int)bits.size(), &d.rkbits);
+    }
     r |= upload(c, cfg->rkindex, cfg->nrkindex, &d.rkindex);
     r |= upload(c, cfg->mobility, cfg->nmobility, &d.mob);
     r |= upload(c, cfg->material, cfg->nmat, &d.mat);

@@ -9,9 +9,11 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
-SOURCES = ["rhs_nc1.hip", "rhs_nc2.hip", "rhs_nc3.hip", "rhs_nc4.hip", "rhs_nc5.hip",
-           "rhs_batch_nc1.hip", "rhs_batch_nc2.hip", "rhs_batch_nc3.hip", "rhs_batch_nc4.hip", "rhs_batch_nc5.hip",
-           "rhs.hip", "post.hip", "ts.hip", "api.hip", "integrator.hip", "batch.hip"]
+# (source, defines): the right-hand-side kernels are instantiated per
+# component count and lane capacity in translation units of their own
+SOURCES = [(src, {"CIPHER_NC": nc, "CIPHER_G": g})
+           for src in ("rhs_inst.hip", "rhs_batch_inst.hip") for nc in (1, 2, 3, 4, 5) for g in (16, 8, 4, 0)]
+SOURCES += [(s, {}) for s in ("rhs.hip", "post.hip", "ts.hip", "api.hip", "integrator.hip", "batch.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -28,24 +30,28 @@ def _stale(src, obj):
 
 
 def build(verbose=False, jobs=8, force=False):
+    import concurrent.futures as cf
     objdir = os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
-    procs = []
+    todo = []
     objs = []
-    for s in SOURCES:
+    for s, defs in SOURCES:
         src = os.path.join(HERE, s)
-        obj = os.path.join(objdir, s + ".o")
+        tag = "".join(f"_{k.split('_')[-1].lower()}{v}" for k, v in defs.items())
+        obj = os.path.join(objdir, s + tag + ".o")
         objs.append(obj)
         if not force and not _stale(src, obj):
             continue
-        cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
+        cmd = [HIPCC] + FLAGS + [f"-D{k}={v}" for k, v in defs.items()] + ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
-        if len(procs) >= jobs:
-            _wait(procs.pop(0))
-    for p in procs:
-        _wait(p)
+        todo.append((s + tag, cmd))
+    # a new compile starts as soon as any finishes (the instantiation units
+    # differ several-fold in compile time)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as pool:
+        for name, r in zip([t[0] for t in todo], pool.map(
+                lambda t: subprocess.run(t[1], stdout=subprocess.PIPE, stderr=subprocess.STDOUT), todo)):
+            _report(name, r)
     out = os.path.join(PKG, "libcipher_amd.so")
     tmp = out + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
@@ -56,11 +62,9 @@ def build(verbose=False, jobs=8, force=False):
     return out
 
 
-def _wait(item):
-    name, p = item
-    out, _ = p.communicate()
-    txt = out.decode(errors="replace")
-    if p.returncode:
+def _report(name, r):
+    txt = r.stdout.decode(errors="replace")
+    if r.returncode:
         raise RuntimeError(f"hipcc {name} failed:\n{txt}")
     warn = [l for l in txt.splitlines() if "warning" in l and "packed-fp32" not in l]
     if warn:

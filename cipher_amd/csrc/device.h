@@ -44,7 +44,19 @@ struct DevCfg {
     int ntseries;
     // T-series values folded at one temperature: tsv[i] = SumTSeries(tsv[ntseries], ts[i])
     const double *tsv;
+    // bit r set: Redlich-Kister entry rk[r] has terms (n > 0); two zero words
+    // of padding after the last entry
+    const uint64_t *rkbits;
 };
+
+// bits [off, off + 64) of rkbits: which of the (at most 64) entries of an RK
+// block starting at rk[off] have terms, so the loops over a block skip empty
+// entries without loading them (they add nothing to any sum)
+__device__ __forceinline__ uint64_t rk_window(const DevCfg &c, int off) {
+    const int w = off >> 6, b = off & 63;
+    const uint64_t lo = c.rkbits[w];
+    return b ? (lo >> b) | (c.rkbits[w + 1] << (64 - b)) : lo;
+}
 
 template <class R> __device__ __forceinline__ R dexp(R x);
 template <> __device__ __forceinline__ double dexp<double>(double x) { return exp(x); }
@@ -284,16 +296,18 @@ __device__ inline void d_mobility(const DevCfg &c, const cipher_material &m, R *
 #pragma unroll
         for (int k = 0; k < NC; k++) {
             const cipher_mobility mb = c.mob[m.mobility + k];
-            const cipher_rk *bin = &c.rk[mb.binary];
+            const uint64_t nz = rk_window(c, mb.binary);
+            int r = 0;
             R mig = R(0);
 #pragma unroll
             for (int j = 0; j < NC; j++) {
                 mig += y[j] * dts<R>(c, mb.unary + j, T);
 #pragma unroll
-                for (int i = j + 1; i < NC; i++, bin++) {
-                    const int nrk = bin->n;
-                    for (int o = 0; o < nrk; o++)
-                        mig += y[j] * y[i] * dts<R>(c, bin->ts + o, T) * dfastpow(y[j] - y[i], (unsigned)o);
+                for (int i = j + 1; i < NC; i++, r++) {
+                    if (!((nz >> r) & 1)) continue;
+                    const cipher_rk bin = c.rk[mb.binary + r];
+                    for (int o = 0; o < bin.n; o++)
+                        mig += y[j] * y[i] * dts<R>(c, bin.ts + o, T) * dfastpow(y[j] - y[i], (unsigned)o);
                 }
             }
             mob[k] = y[k] * (R)mb.m0 / (R)CRG / T * dexp<R>(mig / (R)CRG / T);
@@ -316,62 +330,93 @@ __device__ inline R d_chemenergy(const DevCfg &c, const cipher_material &m, cons
         for (int cp = 0; cp < NC; cp++)
 #pragma unroll
             for (int cq = 0; cq < NC; cq++) e += dts<R>(c, m.unary + cp * NC + cq, T) * yp[cp] * yq[cq];
-        const cipher_rk *rk = &c.rk[m.binary];
-#pragma unroll
-        for (int i = 0; i < NC; i++)
-#pragma unroll
-            for (int j = i + 1; j < NC; j++)
-#pragma unroll
-                for (int cq = 0; cq < NC; cq++, rk++)
-                    for (int o = 0; o < rk->n; o++)
-                        e += dts<R>(c, rk->ts + o, T) * yp[i] * yp[j] * yq[cq] * dfastpow(yp[i] - yp[j], (unsigned)o);
-        rk = &c.rk[m.binaryq];
-#pragma unroll
-        for (int cp = 0; cp < NC; cp++)
-#pragma unroll
-            for (int i = 0; i < NC; i++)
-#pragma unroll
-                for (int j = i + 1; j < NC; j++, rk++)
-                    for (int o = 0; o < rk->n; o++)
-                        e += dts<R>(c, rk->ts + o, T) * yp[cp] * yq[i] * yq[j] * dfastpow(yq[i] - yq[j], (unsigned)o);
-        rk = &c.rk[m.ternary];
-#pragma unroll
-        for (int i = 0; i < NC; i++)
-#pragma unroll
-            for (int j = i + 1; j < NC; j++)
-#pragma unroll
-                for (int k = j + 1; k < NC; k++)
-#pragma unroll
-                    for (int cq = 0; cq < NC; cq++, rk++)
-                        for (int o = 0; o < rk->n; o++)
-                            e += dts<R>(c, rk->ts + o, T) * yp[i] * yp[j] * yp[k] * yq[cq];
-        rk = &c.rk[m.ternaryq];
-#pragma unroll
-        for (int cp = 0; cp < NC; cp++)
+        {
+            const uint64_t nz = rk_window(c, m.binary);
+            int r = 0;
 #pragma unroll
             for (int i = 0; i < NC; i++)
 #pragma unroll
                 for (int j = i + 1; j < NC; j++)
 #pragma unroll
-                    for (int k = j + 1; k < NC; k++, rk++)
-                        for (int o = 0; o < rk->n; o++)
-                            e += dts<R>(c, rk->ts + o, T) * yq[i] * yq[j] * yq[k] * yp[cp];
+                    for (int cq = 0; cq < NC; cq++, r++) {
+                        if (!((nz >> r) & 1)) continue;
+                        const cipher_rk rv = c.rk[m.binary + r];
+                        for (int o = 0; o < rv.n; o++)
+                            e += dts<R>(c, rv.ts + o, T) * yp[i] * yp[j] * yq[cq] * dfastpow(yp[i] - yp[j], (unsigned)o);
+                    }
+        }
+        {
+            const uint64_t nz = rk_window(c, m.binaryq);
+            int r = 0;
+#pragma unroll
+            for (int cp = 0; cp < NC; cp++)
+#pragma unroll
+                for (int i = 0; i < NC; i++)
+#pragma unroll
+                    for (int j = i + 1; j < NC; j++, r++) {
+                        if (!((nz >> r) & 1)) continue;
+                        const cipher_rk rv = c.rk[m.binaryq + r];
+                        for (int o = 0; o < rv.n; o++)
+                            e += dts<R>(c, rv.ts + o, T) * yp[cp] * yq[i] * yq[j] * dfastpow(yq[i] - yq[j], (unsigned)o);
+                    }
+        }
+        {
+            const uint64_t nz = rk_window(c, m.ternary);
+            int r = 0;
+#pragma unroll
+            for (int i = 0; i < NC; i++)
+#pragma unroll
+                for (int j = i + 1; j < NC; j++)
+#pragma unroll
+                    for (int k = j + 1; k < NC; k++)
+#pragma unroll
+                        for (int cq = 0; cq < NC; cq++, r++) {
+                            if (!((nz >> r) & 1)) continue;
+                            const cipher_rk rv = c.rk[m.ternary + r];
+                            for (int o = 0; o < rv.n; o++)
+                                e += dts<R>(c, rv.ts + o, T) * yp[i] * yp[j] * yp[k] * yq[cq];
+                        }
+        }
+        {
+            const uint64_t nz = rk_window(c, m.ternaryq);
+            int r = 0;
+#pragma unroll
+            for (int cp = 0; cp < NC; cp++)
+#pragma unroll
+                for (int i = 0; i < NC; i++)
+#pragma unroll
+                    for (int j = i + 1; j < NC; j++)
+#pragma unroll
+                        for (int k = j + 1; k < NC; k++, r++) {
+                            if (!((nz >> r) & 1)) continue;
+                            const cipher_rk rv = c.rk[m.ternaryq + r];
+                            for (int o = 0; o < rv.n; o++)
+                                e += dts<R>(c, rv.ts + o, T) * yq[i] * yq[j] * yq[k] * yp[cp];
+                        }
+        }
         return e;
     } else if (m.model == CIPHER_CHEM_CALPHADDIS) {
-        const cipher_rk *bin = &c.rk[m.binary], *ter = &c.rk[m.ternary];
+        const uint64_t nzb = rk_window(c, m.binary), nzt = rk_window(c, m.ternary);
+        int rb = 0, rt = 0;
         R e = dts<R>(c, m.ref, T);
 #pragma unroll
         for (int k = 0; k < NC; k++) {
             e += dts<R>(c, m.unary + k, T) * y[k] + (R)CRG * T * y[k] * dlog(y[k]);
 #pragma unroll
-            for (int j = k + 1; j < NC; j++, bin++) {
-                for (int o = 0; o < bin->n; o++)
-                    e += dts<R>(c, bin->ts + o, T) * y[k] * y[j] * dfastpow(y[k] - y[j], (unsigned)o);
+            for (int j = k + 1; j < NC; j++, rb++) {
+                if ((nzb >> rb) & 1) {
+                    const cipher_rk bin = c.rk[m.binary + rb];
+                    for (int o = 0; o < bin.n; o++)
+                        e += dts<R>(c, bin.ts + o, T) * y[k] * y[j] * dfastpow(y[k] - y[j], (unsigned)o);
+                }
 #pragma unroll
-                for (int i = j + 1; i < NC; i++, ter++)
-                    for (int o = 0; o < ter->n; o++)
-                        e += dts<R>(c, ter->ts + o, T) * y[k] * y[j] * y[i]
-                             * (dsel<NC, R>(y, c.rkindex[ter->idx + o]) + (R(1) - y[k] - y[j] - y[i]) / R(3));
+                for (int i = j + 1; i < NC; i++, rt++) {
+                    if (!((nzt >> rt) & 1)) continue;
+                    const cipher_rk ter = c.rk[m.ternary + rt];
+                    for (int o = 0; o < ter.n; o++)
+                        e += dts<R>(c, ter.ts + o, T) * y[k] * y[j] * y[i]
+                             * (dsel<NC, R>(y, c.rkindex[ter.idx + o]) + (R(1) - y[k] - y[j] - y[i]) / R(3));
+                }
             }
         }
         return e;
@@ -407,100 +452,134 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
                 pp[cp] += a * yq[cq];
                 pq[cq] += a * yp[cp];
             }
-        const cipher_rk *rk = &c.rk[m.binary];
-#pragma unroll
-        for (int i = 0; i < NC; i++)
-#pragma unroll
-            for (int j = i + 1; j < NC; j++)
-#pragma unroll
-                for (int cq = 0; cq < NC; cq++, rk++)
-                    for (int o = 0; o < rk->n; o++) {
-                        a = dts<R>(c, rk->ts + o, T);
-                        b = dfastpow(yp[i] - yp[j], (unsigned)o);
-                        cc = (o == 0) ? R(0) : ((R)o) * dfastpow(yp[i] - yp[j], (unsigned)(o - 1));
-                        pp[i] += a * yp[j] * yq[cq] * (b + cc * yp[i]);
-                        pp[j] += a * yp[i] * yq[cq] * (b - cc * yp[j]);
-                        pq[cq] += a * yp[i] * b * yp[j];
-                    }
-        rk = &c.rk[m.binaryq];
-#pragma unroll
-        for (int cp = 0; cp < NC; cp++)
-#pragma unroll
-            for (int i = 0; i < NC; i++)
-#pragma unroll
-                for (int j = i + 1; j < NC; j++, rk++)
-                    for (int o = 0; o < rk->n; o++) {
-                        a = dts<R>(c, rk->ts + o, T);
-                        b = dfastpow(yq[i] - yq[j], (unsigned)o);
-                        cc = (o == 0) ? R(0) : ((R)o) * dfastpow(yq[i] - yq[j], (unsigned)(o - 1));
-                        pq[i] += a * yq[j] * yp[cp] * (b + cc * yq[i]);
-                        pq[j] += a * yq[i] * yp[cp] * (b - cc * yq[j]);
-                        pp[cp] += a * yq[i] * b * yq[j];
-                    }
-        rk = &c.rk[m.ternary];
-#pragma unroll
-        for (int i = 0; i < NC; i++)
-#pragma unroll
-            for (int j = i + 1; j < NC; j++)
-#pragma unroll
-                for (int k = j + 1; k < NC; k++)
-#pragma unroll
-                    for (int cq = 0; cq < NC; cq++, rk++)
-                        for (int o = 0; o < rk->n; o++) {
-                            a = dts<R>(c, rk->ts + o, T);
-                            pp[i] += a * yp[j] * yp[k] * yq[cq];
-                            pp[j] += a * yp[k] * yp[i] * yq[cq];
-                            pp[k] += a * yp[i] * yp[j] * yq[cq];
-                            pq[cq] += a * yp[i] * yp[j] * yp[k];
-                        }
-        rk = &c.rk[m.ternaryq];
-#pragma unroll
-        for (int cp = 0; cp < NC; cp++)
+        {
+            const uint64_t nz = rk_window(c, m.binary);
+            int r = 0;
 #pragma unroll
             for (int i = 0; i < NC; i++)
 #pragma unroll
                 for (int j = i + 1; j < NC; j++)
 #pragma unroll
-                    for (int k = j + 1; k < NC; k++, rk++)
-                        for (int o = 0; o < rk->n; o++) {
-                            a = dts<R>(c, rk->ts + o, T);
-                            pq[i] += a * yq[j] * yq[k] * yp[cp];
-                            pq[j] += a * yq[k] * yq[i] * yp[cp];
-                            pq[k] += a * yq[i] * yq[j] * yp[cp];
-                            pp[cp] += a * yq[i] * yq[j] * yq[k];
+                    for (int cq = 0; cq < NC; cq++, r++) {
+                        if (!((nz >> r) & 1)) continue;
+                        const cipher_rk rv = c.rk[m.binary + r];
+                        for (int o = 0; o < rv.n; o++) {
+                            a = dts<R>(c, rv.ts + o, T);
+                            b = dfastpow(yp[i] - yp[j], (unsigned)o);
+                            cc = (o == 0) ? R(0) : ((R)o) * dfastpow(yp[i] - yp[j], (unsigned)(o - 1));
+                            pp[i] += a * yp[j] * yq[cq] * (b + cc * yp[i]);
+                            pp[j] += a * yp[i] * yq[cq] * (b - cc * yp[j]);
+                            pq[cq] += a * yp[i] * b * yp[j];
                         }
+                    }
+        }
+        {
+            const uint64_t nz = rk_window(c, m.binaryq);
+            int r = 0;
+#pragma unroll
+            for (int cp = 0; cp < NC; cp++)
+#pragma unroll
+                for (int i = 0; i < NC; i++)
+#pragma unroll
+                    for (int j = i + 1; j < NC; j++, r++) {
+                        if (!((nz >> r) & 1)) continue;
+                        const cipher_rk rv = c.rk[m.binaryq + r];
+                        for (int o = 0; o < rv.n; o++) {
+                            a = dts<R>(c, rv.ts + o, T);
+                            b = dfastpow(yq[i] - yq[j], (unsigned)o);
+                            cc = (o == 0) ? R(0) : ((R)o) * dfastpow(yq[i] - yq[j], (unsigned)(o - 1));
+                            pq[i] += a * yq[j] * yp[cp] * (b + cc * yq[i]);
+                            pq[j] += a * yq[i] * yp[cp] * (b - cc * yq[j]);
+                            pp[cp] += a * yq[i] * b * yq[j];
+                        }
+                    }
+        }
+        {
+            const uint64_t nz = rk_window(c, m.ternary);
+            int r = 0;
+#pragma unroll
+            for (int i = 0; i < NC; i++)
+#pragma unroll
+                for (int j = i + 1; j < NC; j++)
+#pragma unroll
+                    for (int k = j + 1; k < NC; k++)
+#pragma unroll
+                        for (int cq = 0; cq < NC; cq++, r++) {
+                            if (!((nz >> r) & 1)) continue;
+                            const cipher_rk rv = c.rk[m.ternary + r];
+                            for (int o = 0; o < rv.n; o++) {
+                                a = dts<R>(c, rv.ts + o, T);
+                                pp[i] += a * yp[j] * yp[k] * yq[cq];
+                                pp[j] += a * yp[k] * yp[i] * yq[cq];
+                                pp[k] += a * yp[i] * yp[j] * yq[cq];
+                                pq[cq] += a * yp[i] * yp[j] * yp[k];
+                            }
+                        }
+        }
+        {
+            const uint64_t nz = rk_window(c, m.ternaryq);
+            int r = 0;
+#pragma unroll
+            for (int cp = 0; cp < NC; cp++)
+#pragma unroll
+                for (int i = 0; i < NC; i++)
+#pragma unroll
+                    for (int j = i + 1; j < NC; j++)
+#pragma unroll
+                        for (int k = j + 1; k < NC; k++, r++) {
+                            if (!((nz >> r) & 1)) continue;
+                            const cipher_rk rv = c.rk[m.ternaryq + r];
+                            for (int o = 0; o < rv.n; o++) {
+                                a = dts<R>(c, rv.ts + o, T);
+                                pq[i] += a * yq[j] * yq[k] * yp[cp];
+                                pq[j] += a * yq[k] * yq[i] * yp[cp];
+                                pq[k] += a * yq[i] * yq[j] * yp[cp];
+                                pp[cp] += a * yq[i] * yq[j] * yq[k];
+                            }
+                        }
+        }
     } else if (m.model == CIPHER_CHEM_CALPHADDIS) {
         R a, b, cc;
 #pragma unroll
         for (int i = 0; i < NC; i++) full[i] = dts<R>(c, m.unary + i, T);
-        const cipher_rk *rk = &c.rk[m.binary];
+        {
+            const uint64_t nz = rk_window(c, m.binary);
+            int r = 0;
 #pragma unroll
-        for (int i = 0; i < NC; i++)
+            for (int i = 0; i < NC; i++)
 #pragma unroll
-            for (int j = i + 1; j < NC; j++, rk++)
-                for (int o = 0; o < rk->n; o++) {
-                    a = dts<R>(c, rk->ts + o, T);
-                    b = dfastpow(y[i] - y[j], (unsigned)o);
-                    cc = (o == 0) ? R(0) : ((R)o) * dfastpow(y[i] - y[j], (unsigned)(o - 1));
-                    full[i] += a * y[j] * (b + cc * y[i]);
-                    full[j] += a * y[i] * (b - cc * y[j]);
+                for (int j = i + 1; j < NC; j++, r++) {
+                    if (!((nz >> r) & 1)) continue;
+                    const cipher_rk rv = c.rk[m.binary + r];
+                    for (int o = 0; o < rv.n; o++) {
+                        a = dts<R>(c, rv.ts + o, T);
+                        b = dfastpow(y[i] - y[j], (unsigned)o);
+                        cc = (o == 0) ? R(0) : ((R)o) * dfastpow(y[i] - y[j], (unsigned)(o - 1));
+                        full[i] += a * y[j] * (b + cc * y[i]);
+                        full[j] += a * y[i] * (b - cc * y[j]);
+                    }
                 }
-        rk = &c.rk[m.ternary];
+        }
+        const uint64_t nzt = rk_window(c, m.ternary);
+        int rt = 0;
 #pragma unroll
         for (int i = 0; i < NC; i++)
 #pragma unroll
             for (int j = i + 1; j < NC; j++)
 #pragma unroll
-                for (int k = j + 1; k < NC; k++, rk++)
-                    for (int o = 0; o < rk->n; o++) {
-                        const int ix = c.rkindex[rk->idx + o];
-                        a = dts<R>(c, rk->ts + o, T);
+                for (int k = j + 1; k < NC; k++, rt++) {
+                    if (!((nzt >> rt) & 1)) continue;
+                    const cipher_rk rv = c.rk[m.ternary + rt];
+                    for (int o = 0; o < rv.n; o++) {
+                        const int ix = c.rkindex[rv.idx + o];
+                        a = dts<R>(c, rv.ts + o, T);
                         b = dsel<NC, R>(y, ix) + (R(1) - y[k] - y[j] - y[i]) / R(3);
                         full[i] += a * y[j] * y[k] * (b - y[i] / R(3));
                         full[j] += a * y[k] * y[i] * (b - y[j] / R(3));
                         full[k] += a * y[i] * y[j] * (b - y[k] / R(3));
                         daddat<NC, R>(full, ix, a * y[i] * y[j] * y[k]);
                     }
+                }
     }
     const int ns = m.nsites;
 #pragma unroll

@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) void k_prep_batch_lists(const BatchMember *__r
 }
 
 template <int G, int NC>
-static cipher_status batch_G(const BatchLaunch &L, int precision, hipStream_t st) {
+cipher_status batch_G(const BatchLaunch &L, int precision, hipStream_t st) {
     if (L.do_bin && L.bin.nblocks)
         hipLaunchKernelGGL(k_bin_batch<0>, dim3((unsigned)L.bin.nblocks), dim3(256), 0, st, L.mem, L.live, L.bin.seg,
                            L.bin.segmem, L.bin.nseg, L.nbins, L.X);

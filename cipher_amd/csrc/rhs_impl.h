@@ -1402,7 +1402,7 @@ static inline ListRanges make_ranges(const ListShape *shape, long long N, const 
 
 // ---------------------------------------------------------------------------
 template <int G, int NC>
-static cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st) {
+cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st) {
     constexpr int CPB = 256 / G;
     const long long N = L.ncells;
     const long long nb = (N + CPB - 1) / CPB;

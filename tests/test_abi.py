@@ -106,3 +106,36 @@ def test_runtime_refuses_to_run_without_the_library(monkeypatch, tmp_path):
     monkeypatch.setattr(abi, "_lib", None)
     with pytest.raises(ImportError):
         abi.load_library(str(tmp_path / "missing.so"))
+
+
+def test_config_validation_accepts_every_workload_and_rejects_bad_rk_blocks(lib):
+    """cipher_ctx_create validates the tables before any device call: every
+    workload config of the tests passes (without a device the call then fails
+    with a HIP error, not an argument error), and a material whose
+    Redlich-Kister block runs past the table is refused."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is visible")
+    from cipher_amd import config as cfgmod
+    from cipher_amd import workload as W
+    ctx = C.c_void_p()
+    cfgs = [H.chem().cfg, H.chem(thermal=True).cfg, H.ternary().cfg]
+    cfgs += [H.ref_case(n).cfg for n in H.REF_CASES]
+    for sp in W.ensemble_specs(96)[:12]:
+        if sp["components"] > 1:
+            cfgs.append(cfgmod.load_config(W.solution_2d_yaml(res=(64, 64), ngrains=10, ncomp=sp["components"],
+                                                              ntypes=sp["types"], seed=sp["seed"])))
+    for cfg in cfgs:
+        rc = lib.cipher_ctx_create(C.byref(cfg.struct), abi.FP64, C.byref(ctx))
+        assert rc == 2, (rc, lib.cipher_last_error())
+    cfg = H.ternary().cfg
+    m = [i for i, mt in enumerate(cfg.materials) if mt.model == abi.CHEM_CALPHAD2SL][0]
+    mats = (abi.Material * cfg.struct.nmat)()
+    for i in range(cfg.struct.nmat):
+        mats[i] = cfg.struct.material[i]
+    mats[m].ternaryq = cfg.struct.nrk - 1
+    bad = abi.Config()
+    C.pointer(bad)[0] = cfg.struct
+    bad.material = C.cast(mats, C.POINTER(abi.Material))
+    rc = lib.cipher_ctx_create(C.byref(bad), abi.FP64, C.byref(ctx))
+    assert rc == abi.ERR_ARG and b"Redlich-Kister" in lib.cipher_last_error()

@@ -1,0 +1,295 @@
+"""HIP path vs the CPU oracle at the full sizes of the benchmark
+configurations, through the C ABI:
+
+* configuration 2: 3D AMR grain growth, 64^3 domain, levels 3-6, 500 grains,
+  energies U[0.8, 1.2] (230 294 cells), capacity 8;
+* configurations 3 and 4: 3D AMR ternary grain-boundary precipitation,
+  128^3 domain, levels 4-7, 100 grains and 40 precipitates (848 835 cells),
+  in FP64 and in FP32-mixed;
+* configuration 5: all 96 members of the batched 2D ensemble (7.07 M cells)
+  through one batch, in FP64 and in FP32-mixed.
+
+States: SetUpProblem (chemical potential uniform from the matrix) and the
+benchmark's evolved state (explicit phase-field steps with PostStep,
+bench.prepare_state).  Bars: active flags equal to the oracle's; FP64
+residual per field within 1e-12 relative, or 4x the oracle's own change
+under one-ulp perturbations of the state where that is larger (grain growth:
+identical bits); FP32-mixed phase-fraction rate within 1e-6 relative of the
+oracle, the other fields within the bar their FP64 conditioning allows (see
+_check32).  The per-field errors are printed.
+"""
+import concurrent.futures as cf
+
+import numpy as np
+import pytest
+
+import oracle
+from cipher_amd import abi
+from cipher_amd import workload as W
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+RTOL64 = 1e-12
+RTOL32_PF = 1e-6
+
+
+def _bench():
+    import bench
+    return bench
+
+
+def _host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _gg3d(capacity=8):
+    txt = W.grain_growth_yaml(res=(64, 64, 64), ngrains=500, periodic=False, energy_range=(0.8, 1.2),
+                              ntypes=255, minlevel=3, t_end=50.0, capacity=capacity)
+    return W.Workload(txt, f"grain_growth_3d_amr_64_cap{capacity}")
+
+
+def _ternary():
+    txt = W.ternary_ppt_yaml(res=(128, 128, 128), ngrains=100, nppt=40, radius=3, minlevel=4)
+    return W.Workload(txt, "ternary_gb_precipitation_128")
+
+
+def _rhs(ctx, X):
+    import torch
+    act = torch.zeros(ctx.ncells, dtype=torch.int32, device=ctx.device)
+    F = ctx.rhs(X, active=act)
+    return _host(F), _host(act).astype(np.uint32)
+
+
+def _check64(label, wl, X, Fg, ag, Fo, ao, exact=False):
+    """FP64 against the oracle; returns the oracle's one-ulp spread."""
+    assert np.array_equal(ag, ao), f"{label}: active flags differ in {np.count_nonzero(ag != ao)} cells"
+    if exact:
+        assert np.array_equal(Fg, Fo), f"{label}: max diff {np.abs(Fg - Fo).max():.3e}"
+        print(f"   {label}: FP64 identical bits", flush=True)
+        return None
+    errs = H.field_errors(Fg, Fo, wl.layout)
+    spread = H.ulp_spread(wl, X, Fo, trials=1)
+    print(f"   {label}: FP64 errors {errs}, oracle one-ulp spread {spread}", flush=True)
+    for f, e in errs.items():
+        bar = max(RTOL64, 4.0 * spread[f])
+        assert e <= bar, f"{label} field {f}: rel err {e:.3e} > {bar:.3e}"
+    return spread
+
+
+def _check32(label, wl, F32, a32, Fo, ao, spread=None):
+    """FP32-mixed against the oracle: active flags equal, phase-fraction rate
+    within 1e-6 relative; every other field within max(1e-6, 4 s 2^29), s
+    its FP64 one-ulp spread: a field that moves by s under 2^-53 relative
+    input changes moves by about s 2^29 under the 2^-24 rounding of FP32
+    inputs, whatever the evaluation (the chemical-potential rate of a
+    single-material member at uniform mu is 0 in exact arithmetic, so its
+    FP64 value is rounding noise and its spread is of order 1)."""
+    assert np.array_equal(a32, ao), f"{label}: FP32-mixed active flags differ in {np.count_nonzero(a32 != ao)} cells"
+    errs = H.field_errors(F32, Fo, wl.layout)
+    print(f"   {label}: FP32-mixed errors vs oracle {errs}", flush=True)
+    assert errs["PF"] <= RTOL32_PF, f"{label}: FP32-mixed phase-fraction rate rel err {errs['PF']:.3e}"
+    if spread is not None:
+        for f, e in errs.items():
+            if f == "PF":
+                continue
+            bar = max(RTOL32_PF, 4.0 * spread[f] * 2.0 ** 29)
+            assert e <= bar, f"{label} field {f}: FP32-mixed rel err {e:.3e} > {bar:.3e}"
+    return errs
+
+
+def _diffuse(wl, X, seed=0, amp=0.4):
+    """Random phase fractions on every cell's active slots, projected onto
+    the simplex (active sets unchanged, no union step)."""
+    L = wl.layout
+    D = L["ndof"]
+    rng = np.random.default_rng(seed)
+    Xv = X.reshape(-1, D).copy()
+    n = np.rint(Xv[:, 0]).astype(int)
+    for k in np.unique(n):
+        if k < 2:
+            continue
+        sel = np.flatnonzero(n == k)
+        v = Xv[sel, L["pf_offset"]:L["pf_offset"] + k] + amp * rng.random((sel.size, k))
+        u = -np.sort(-v, axis=1)
+        css = np.cumsum(u, axis=1) - 1.0
+        idx = np.arange(1, k + 1)
+        rho = (u - css / idx > 0).sum(axis=1)
+        tau = css[np.arange(sel.size), rho - 1] / rho
+        Xv[sel, L["pf_offset"]:L["pf_offset"] + k] = np.maximum(v - tau[:, None], 0.0)
+    return Xv.ravel()
+
+
+# --------------------------------------------------------------------------
+# configuration 2
+# --------------------------------------------------------------------------
+@pytest.mark.timeout(600)
+def test_config2_grain_growth_3d_full_size():
+    import torch
+    from cipher_amd.runtime import Context
+    wl = _gg3d()
+    print(f"[config2] {wl.ncells} cells, {wl.mesh.nfaces} faces", flush=True)
+    c64 = Context(wl.cfg, precision=abi.FP64)
+    c64.upload_mesh(wl.mesh)
+    c32 = Context(wl.cfg, precision=abi.FP32_MIXED)
+    c32.upload_mesh(wl.mesh)
+    Xg = c64.setup_problem(wl.phase)
+    X0 = oracle.setup_problem(wl.cfg, wl.mesh, wl.phase)
+    m = H.state_mask(X0, wl.layout, wl.cfg)
+    assert np.array_equal(_host(Xg)[m], X0[m]), "SetUpProblem differs"
+    for sname, X in (("init", X0), ("diffuse", _diffuse(wl, X0))):
+        Xd = torch.as_tensor(X).to(c64.device)
+        Fo, ao = oracle.rhs(wl.cfg, wl.mesh, X, want_active=True)
+        Fg, ag = _rhs(c64, Xd)
+        _check64(f"config2/{sname}", wl, X, Fg, ag, Fo, ao, exact=True)
+        F32, a32 = _rhs(c32, Xd)
+        _check32(f"config2/{sname}", wl, F32, a32, Fo, ao)
+    # capacity 8 does not hold this microstructure: the first explicit step
+    # makes unions of 9 phases, which the reference asserts on
+    # (src/cipher.c:968); device and oracle report the same overflow
+    X = X0 + 0.02 / np.abs(Fo).max() * Fo
+    _, rc = oracle.poststep(wl.cfg, wl.mesh, X)
+    assert rc == abi.ERR_CAPACITY
+    with pytest.raises(abi.CipherError) as e:
+        c64.poststep(torch.as_tensor(X).to(c64.device))
+    assert e.value.code == abi.ERR_CAPACITY
+    c64.close()
+    c32.close()
+
+
+@pytest.mark.timeout(900)
+def test_config2_trajectory_at_capacity_16():
+    """The adaptive Bogacki-Shampine run of configuration 2 from SetUpProblem
+    for a fixed number of accepted steps, at capacity 16 (capacity 8 holds
+    no step of this microstructure): device against the CPU restatement
+    (oracle right-hand side, the same controller), equal accepted and
+    rejected counts and step sizes, final state within 1e-9."""
+    import torch
+    from cipher_amd.runtime import Context
+    wl = _gg3d(capacity=16)
+    p = abi.ts_params(wl.cfg.abstol, wl.cfg.reltol, wl.cfg.timestepmin, wl.cfg.timestepmax, use_graph=True)
+    X0 = oracle.setup_problem(wl.cfg, wl.mesh, wl.phase)
+    steps = 6
+    print(f"[config2 traj] {wl.ncells} cells, capacity 16, {steps} accepted steps", flush=True)
+    Xc, info = H.cpu_solve(wl, X0, 0.0, 50.0, wl.cfg.timestepmin, p, max_steps=steps)
+    print(f"   cpu steps={info['steps']} rejected={info['rejected']} t={info['t']:.6e} dt={info['dt']:.6e}",
+          flush=True)
+    ctx = Context(wl.cfg)
+    ctx.upload_mesh(wl.mesh)
+    ctx.ts_setup(p)
+    Xd = torch.as_tensor(X0).to(ctx.device)
+    st, dt = ctx.ts_solve(Xd, 0.0, 50.0, wl.cfg.timestepmin, max_steps=steps)
+    print(f"   gpu steps={st.steps} rejected={st.rejected} t={st.t:.6e} dt={dt:.6e}", flush=True)
+    assert (st.steps, st.rejected) == (info["steps"], info["rejected"])
+    assert st.t == pytest.approx(info["t"], rel=1e-12, abs=0.0)
+    assert dt == pytest.approx(info["dt"], rel=1e-8, abs=0.0)
+    Xg = _host(Xd)
+    L = wl.layout
+    m = H.state_mask(Xc, L, wl.cfg)
+    assert np.array_equal(H.state_mask(Xg, L, wl.cfg), m), "active sets differ"
+    errs = H.field_errors(np.where(m, Xg, 0.0), np.where(m, Xc, 0.0), L)
+    n = np.rint(Xg.reshape(-1, L["ndof"])[:, 0]).astype(int)
+    print(f"   final errors {errs}, active-set sizes {np.bincount(n).tolist()}", flush=True)
+    assert errs["PF"] <= 1e-9
+    ctx.close()
+
+
+# --------------------------------------------------------------------------
+# configurations 3 and 4
+# --------------------------------------------------------------------------
+@pytest.mark.timeout(900)
+def test_config3_and_4_ternary_full_size():
+    from cipher_amd.runtime import Context
+    b = _bench()
+    wl = _ternary()
+    print(f"[config3/4] {wl.ncells} cells, {wl.mesh.nfaces} faces", flush=True)
+    c64 = Context(wl.cfg, precision=abi.FP64)
+    c64.upload_mesh(wl.mesh)
+    c32 = Context(wl.cfg, precision=abi.FP32_MIXED)
+    c32.upload_mesh(wl.mesh)
+    for steps in (0, 20):
+        Xd, ev, nact = b.prepare_state(c64, wl, steps)
+        X = _host(Xd)
+        label = f"config3/evolved{ev}"
+        print(f"[{label}] active-set sizes {nact.tolist()}", flush=True)
+        Fo, ao = oracle.rhs(wl.cfg, wl.mesh, X, want_active=True)
+        Fg, ag = _rhs(c64, Xd)
+        spread = _check64(label, wl, X, Fg, ag, Fo, ao)
+        F32, a32 = _rhs(c32, Xd)
+        _check32(f"config4/evolved{ev}", wl, F32, a32, Fo, ao, spread)
+    c64.close()
+    c32.close()
+
+
+# --------------------------------------------------------------------------
+# configuration 5
+# --------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def ensemble():
+    import torch
+    from cipher_amd.runtime import Batch, Context
+    b = _bench()
+    specs = W.ensemble_specs(n=96)
+    wls, ctxs, states = [], [], []
+    for i, sp in enumerate(specs):
+        wl = W.ensemble_member(sp)
+        ctx = Context(wl.cfg, precision=abi.FP64)
+        ctx.upload_mesh(wl.mesh)
+        X, _, _ = b.prepare_state(ctx, wl, 20)
+        wls.append(wl)
+        ctxs.append(ctx)
+        states.append(X)
+        if i % 16 == 15:
+            print(f"[config5] {i + 1}/96 members ready", flush=True)
+    X = torch.cat(states)
+    del states
+    bt = Batch(ctxs)
+    out = {}
+    for prec in (abi.FP64, abi.FP32_MIXED):
+        bt.set_precision(prec)
+        act = torch.zeros(bt.cell_offsets[-1], dtype=torch.int32, device=X.device)
+        F = bt.rhs(X, active=act)
+        out[prec] = (_host(F), _host(act).astype(np.uint32))
+    Xh = _host(X)
+    res = dict(wls=wls, X=Xh, offsets=list(bt.offsets), cell_offsets=list(bt.cell_offsets), F=out)
+    bt.close()
+    for c in ctxs:
+        c.close()
+    return res
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("chunk", range(8))
+def test_config5_ensemble_full_size(ensemble, chunk):
+    e = ensemble
+    members = list(range(12 * chunk, 12 * chunk + 12))
+
+    def one(i):
+        wl = e["wls"][i]
+        sl = slice(e["offsets"][i], e["offsets"][i + 1])
+        cs = slice(e["cell_offsets"][i], e["cell_offsets"][i + 1])
+        X = e["X"][sl]
+        Fo, ao = oracle.rhs(wl.cfg, wl.mesh, X, want_active=True)
+        spread = H.ulp_spread(wl, X, Fo, trials=1)
+        return i, X, Fo, ao, spread, cs, sl
+
+    with cf.ThreadPoolExecutor(max_workers=8) as pool:
+        results = list(pool.map(one, members))
+    worst32 = {}
+    for i, X, Fo, ao, spread, cs, sl in results:
+        wl = e["wls"][i]
+        F64, a64 = e["F"][abi.FP64]
+        F32, a32 = e["F"][abi.FP32_MIXED]
+        Fg, ag = F64[sl], a64[cs]
+        assert np.array_equal(ag, ao), f"{wl.name}: active flags differ"
+        errs = H.field_errors(Fg, Fo, wl.layout)
+        for f, err in errs.items():
+            bar = max(RTOL64, 4.0 * spread[f])
+            assert err <= bar, f"{wl.name} field {f}: FP64 rel err {err:.3e} > {bar:.3e}"
+        e32 = _check32(wl.name, wl, F32[sl], a32[cs], Fo, ao, spread)
+        for f, v in e32.items():
+            worst32[f] = max(worst32.get(f, 0.0), v)
+        print(f"   {wl.name}: FP64 {errs}", flush=True)
+    print(f"[config5 chunk {chunk}] worst FP32-mixed error per field {worst32}", flush=True)
