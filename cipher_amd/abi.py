@@ -117,7 +117,7 @@ class TsCtrl(C.Structure):
 
 class TsStats(C.Structure):
     _fields_ = [("steps", C.c_int64), ("rejected", C.c_int64), ("rhs_evals", C.c_int64), ("t", C.c_double),
-                ("dt", C.c_double), ("err", C.c_double)]
+                ("dt", C.c_double), ("err", C.c_double), ("host_syncs", C.c_int64)]
 
 
 OUT_PHASEID, OUT_MATID, OUT_INTERFACEID, OUT_CHEMPOT, OUT_COMP, OUT_PHI, OUT_TEMPERATURE = range(1, 8)

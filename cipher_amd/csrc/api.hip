@@ -470,6 +470,7 @@ int cipher_rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active,
     L.ncells = c->ncells;
     for (int q = 0; q < 5; q++) L.shape[q] = c->shape[q];
     L.bins_mode = bins_mode;
+    L.pred = c->pred;
     if (c->fold) {
         cipher_status f = launch_tsfold(X, c->ncells, c->dc, c->tsv, c->stream);
         if (f != CIPHER_OK) return fail(f, "fold launch");

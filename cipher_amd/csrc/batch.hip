@@ -65,6 +65,12 @@ struct BatchTs {
     // per-member step-controller memory, carried from call to call as a
     // context's is (cleared by cipher_batch_ts_setup)
     std::vector<cipher_ts_ctrl> ctrl;
+    // device-driven solve: per-member loop state and the captured step
+    TsDev *dev = nullptr;
+    TsDev *h_dev = nullptr;           // pinned, nmembers
+    hipGraph_t sgraph = nullptr;
+    hipGraphExec_t sexec = nullptr;
+    const double *sgraph_X = nullptr;
 };
 
 }  // namespace
@@ -231,6 +237,11 @@ static void ts_invalidate(cipher_batch *b) {
     s.exec = nullptr;
     s.graph = nullptr;
     s.graph_X = nullptr;
+    if (s.sexec) (void)hipGraphExecDestroy(s.sexec);
+    if (s.sgraph) (void)hipGraphDestroy(s.sgraph);
+    s.sexec = nullptr;
+    s.sgraph = nullptr;
+    s.sgraph_X = nullptr;
 }
 
 static void ts_free(cipher_batch *b) {
@@ -241,6 +252,10 @@ static void ts_free(cipher_batch *b) {
         if (p) (void)hipFree(p);
     for (int i = 0; i < 4; i++) s.K[i] = nullptr;
     s.Y = s.Xn = s.Xh = s.dt_dev = nullptr;
+    if (s.dev) (void)hipFree(s.dev);
+    if (s.h_dev) (void)hipHostFree(s.h_dev);
+    s.dev = nullptr;
+    s.h_dev = nullptr;
     if (s.own) (void)hipStreamDestroy(s.own);
     s.own = nullptr;
     s.ready = false;
@@ -627,6 +642,8 @@ extern "C" int cipher_batch_ts_setup(cipher_batch *b, const cipher_ts_params *p)
     BCHK(hipMalloc(&s.Xn, bytes));
     BCHK(hipMalloc(&s.Xh, bytes));
     BCHK(hipMalloc(&s.dt_dev, sizeof(double) * b->m.size()));
+    BCHK(hipMalloc(&s.dev, sizeof(TsDev) * b->m.size()));
+    BCHK(hipHostMalloc(&s.h_dev, sizeof(TsDev) * b->m.size(), hipHostMallocDefault));
     s.ctrl.assign(b->m.size(), cipher_ts_ctrl{});
     s.ready = true;
     return CIPHER_OK;
@@ -676,6 +693,60 @@ static int capture(cipher_batch *b, const double *X) {
     return CIPHER_OK;
 }
 
+// ---- device-driven steps: one thread per member ----------------------------
+__global__ void k_bts_begin(TsDev *s, int n, double *dt_dev, const int32_t *err, int32_t *live) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) ts_begin(s[i], err[i], dt_dev + i, live + i);
+}
+
+__global__ void k_bts_decide(TsDev *s, int n, const double *norms, cipher_ts_params p, int32_t *acc) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) ts_decide(s[i], p, norms[i], acc + i);
+}
+
+// one step of every member on the device: per-member begin, the attempt
+// (members that do not run skip every launch), per-member decision, copy
+// and PostStep of the accepted members
+static int enqueue_step(cipher_batch *b, double *X) {
+    BatchTs &s = b->ts;
+    hipStream_t st = b->stream;
+    const int n = (int)b->m.size();
+    const unsigned nb = (unsigned)((n + 63) / 64);
+    hipLaunchKernelGGL(k_bts_begin, dim3(nb), dim3(64), 0, st, s.dev, n, s.dt_dev, (const int32_t *)b->d_err,
+                       b->d_live);
+    int rc = enqueue_attempt(b, X);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bts_decide, dim3(nb), dim3(64), 0, st, s.dev, n, (const double *)b->d_norms, s.p, b->d_acc);
+    if (launch_copy_batch(b->all, X, s.Xn, b->d_acc, st)) return cipher_fail(CIPHER_ERR_HIP, "accept copy");
+    for (BatchClass &k : b->cls) {
+        k.L.live = b->d_acc;
+        if (launch_post_batch(k.L, X, st)) return cipher_fail(CIPHER_ERR_HIP, "batch poststep launch");
+    }
+    BCHK(hipGetLastError());
+    return CIPHER_OK;
+}
+
+static int capture_step(cipher_batch *b, double *X) {
+    BatchTs &s = b->ts;
+    if (s.sexec) (void)hipGraphExecDestroy(s.sexec);
+    if (s.sgraph) (void)hipGraphDestroy(s.sgraph);
+    s.sexec = nullptr;
+    s.sgraph = nullptr;
+    BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+    int rc = enqueue_step(b, X);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return cipher_fail(CIPHER_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    s.sgraph = g;
+    BCHK(hipGraphInstantiate(&s.sexec, g, nullptr, nullptr, 0));
+    s.sgraph_X = X;
+    return CIPHER_OK;
+}
+
 extern "C" int cipher_batch_ts_solve(cipher_batch *b, double *X, const double *t0, const double *t_end,
                                      double *dt_inout, int64_t max_steps, cipher_ts_stats *stats) {
     if (!b || !X || !t0 || !t_end || !dt_inout || !stats) return cipher_fail(CIPHER_ERR_ARG, "null argument");
@@ -690,83 +761,79 @@ extern "C" int cipher_batch_ts_solve(cipher_batch *b, double *X, const double *t
         b->stream = s.own;
         restore = true;
     }
-    std::vector<double> t(t0, t0 + n), dt(n), dt_try(n);
-    std::vector<cipher_ts_ctrl> &ctrl = s.ctrl;
-    std::vector<cipher_ts_stats> out(n);
-    for (int i = 0; i < n; i++) dt[i] = dt_inout[i] > 0.0 ? dt_inout[i] : s.p.dt_min;
-    int rc = CIPHER_OK;
     hipStream_t st = b->stream;
-    for (;;) {
-        int nlive = 0;
-        for (int i = 0; i < n; i++) {
-            bool lv = t[i] < t_end[i] && (max_steps <= 0 || out[i].steps < max_steps) && b->m[i]->ncells > 0;
-            dt_try[i] = (t[i] + dt[i] > t_end[i]) ? t_end[i] - t[i] : dt[i];
-            if (!(dt_try[i] > 0.0)) lv = false;
-            b->h_live[i] = lv ? 1 : 0;
-            b->h_dt[i] = lv ? dt_try[i] : 0.0;
-            nlive += lv ? 1 : 0;
-        }
-        if (!nlive) break;
-        hipError_t e = hipMemcpyAsync(s.dt_dev, b->h_dt, sizeof(double) * n, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(b->d_live, b->h_live, sizeof(int32_t) * n, hipMemcpyHostToDevice, st);
-        if (e != hipSuccess) {
-            rc = cipher_fail(CIPHER_ERR_HIP, "step upload");
-            break;
-        }
-        if (s.p.use_graph) {
-            if (!s.exec || s.graph_X != X) rc = capture(b, X);
-            if (!rc && hipGraphLaunch(s.exec, st) != hipSuccess) rc = cipher_fail(CIPHER_ERR_HIP, "graph launch");
-        } else {
-            rc = enqueue_attempt(b, X);
-        }
-        if (rc) break;
-        e = hipMemcpyAsync(b->h_norms, b->d_norms, sizeof(double) * n, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) {
-            rc = cipher_fail(CIPHER_ERR_HIP, std::string("attempt: ") + hipGetErrorString(e));
-            break;
-        }
-        int nacc = 0;
-        for (int i = 0; i < n && !rc; i++) {
-            b->h_acc[i] = 0;
-            if (!b->h_live[i]) continue;
-            const double err = b->h_norms[i];
-            out[i].rhs_evals += 4;
-            out[i].err = err;
-            int acc = 0;
-            double dtn = dt_try[i];
-            if ((rc = cipher_ts_adapt(&s.p, &ctrl[i], dt_try[i], err, &acc, &dtn))) {
-                rc = cipher_fail(rc, std::string(cipher_last_error()) + " (member " + std::to_string(i) + ")");
-                break;
-            }
-            if (acc) {
-                b->h_acc[i] = 1;
-                nacc++;
-                t[i] = (dt_try[i] == t_end[i] - t[i]) ? t_end[i] : t[i] + dt_try[i];
-                out[i].steps++;
+    for (int i = 0; i < n; i++) {
+        TsDev &h = s.h_dev[i];
+        h = TsDev{};
+        h.t = t0[i];
+        h.t_end = t_end[i];
+        h.dt = dt_inout[i] > 0.0 ? dt_inout[i] : s.p.dt_min;
+        h.ctrl = s.ctrl[i];
+        h.max_steps = max_steps;
+        h.empty = b->m[i]->ncells > 0 ? 0 : 1;
+    }
+    int rc = CIPHER_OK;
+    int64_t syncs = 0;
+    hipError_t e = hipMemcpyAsync(s.dev, s.h_dev, sizeof(TsDev) * n, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) rc = cipher_fail(CIPHER_ERR_HIP, "solve set-up");
+    if (!rc) rc = fill(b, b->d_err, n, 0, st);
+    int group = 1;
+    while (!rc) {
+        for (int k = 0; k < group && !rc; k++) {
+            if (s.p.use_graph) {
+                if (!s.sexec || s.sgraph_X != X) rc = capture_step(b, X);
+                if (!rc && hipGraphLaunch(s.sexec, st) != hipSuccess) rc = cipher_fail(CIPHER_ERR_HIP, "graph launch");
             } else {
-                out[i].rejected++;
+                rc = enqueue_step(b, X);
             }
-            dt[i] = dtn;
         }
         if (rc) break;
-        if (nacc) {
-            if (hipMemcpyAsync(b->d_acc, b->h_acc, sizeof(int32_t) * n, hipMemcpyHostToDevice, st) != hipSuccess) {
-                rc = cipher_fail(CIPHER_ERR_HIP, "accept upload");
-                break;
-            }
-            if (launch_copy_batch(b->all, X, s.Xn, b->d_acc, st)) {
-                rc = cipher_fail(CIPHER_ERR_HIP, "accept copy");
-                break;
-            }
-            if ((rc = batch_post_impl(b, X, b->d_acc, nullptr))) break;
+        e = hipMemcpyAsync(s.h_dev, s.dev, sizeof(TsDev) * n, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        syncs++;
+        if (e != hipSuccess) {
+            rc = cipher_fail(CIPHER_ERR_HIP, std::string("step: ") + hipGetErrorString(e));
+            break;
         }
+        // the next group: the attempts the slowest running member needs to
+        // reach its end time at its step size (one more for a rejection),
+        // at most twice the last group and 16
+        bool all_done = true;
+        int want = 1;
+        for (int i = 0; i < n; i++) {
+            const TsDev &h = s.h_dev[i];
+            if (h.done) continue;
+            all_done = false;
+            const double rem = (h.t_end - h.t) / h.dt;
+            int w = rem < 15.0 ? (int)rem + 2 : 16;
+            if (h.max_steps > 0 && h.max_steps - h.steps + 1 < w) w = (int)(h.max_steps - h.steps + 1);
+            want = std::max(want, w);
+        }
+        if (all_done) break;
+        group = std::min(std::min(2 * group, 16), std::max(want, 1));
+    }
+    for (int i = 0; i < n && !rc; i++) {
+        const TsDev &h = s.h_dev[i];
+        if (h.status == CIPHER_ERR_CAPACITY)
+            rc = cipher_fail(CIPHER_ERR_CAPACITY,
+                             "active-set union exceeds the capacity PF_SIZE in member " + std::to_string(i));
+        else if (h.status == CIPHER_ERR_NONFINITE)
+            rc = cipher_fail(CIPHER_ERR_NONFINITE,
+                             "error norm is not finite at the minimum step (member " + std::to_string(i) + ")");
     }
     for (int i = 0; i < n; i++) {
-        out[i].t = t[i];
-        out[i].dt = dt[i];
-        stats[i] = out[i];
-        dt_inout[i] = dt[i];
+        const TsDev &h = s.h_dev[i];
+        s.ctrl[i] = h.ctrl;
+        cipher_ts_stats o{};
+        o.steps = h.steps;
+        o.rejected = h.rejected;
+        o.rhs_evals = h.rhs_evals;
+        o.t = h.t;
+        o.dt = h.dt;
+        o.err = h.err;
+        o.host_syncs = syncs;
+        stats[i] = o;
+        dt_inout[i] = h.dt;
     }
     if (restore) b->stream = nullptr;
     return rc;

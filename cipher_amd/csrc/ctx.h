@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "ctrl.h"
 #include "kernels.h"
 
 // One coordinate step of the launch-shape search: set one field of a
@@ -57,6 +58,14 @@ struct TsState {
     hipGraphExec_t exec = nullptr;
     const double *graph_X = nullptr;  // solution pointer the graph was captured for
     hipStream_t own = nullptr;        // capture stream when the context has none
+    // device-driven solve: loop state, the attempt's run flag and the
+    // accept flag, and the captured step (attempt, decision, accept)
+    cipher::TsDev *dev = nullptr;
+    cipher::TsDev *h_dev = nullptr;   // pinned
+    int32_t *live = nullptr, *acc = nullptr;
+    hipGraph_t sgraph = nullptr;
+    hipGraphExec_t sexec = nullptr;
+    const double *sgraph_X = nullptr;
 };
 
 struct cipher_ctx {
@@ -90,6 +99,7 @@ struct cipher_ctx {
     int npart = 1024;
     TsState ts;                     // adaptive integrator (integrator.hip)
     cipher::ListShape shape[5];     // launch shapes of the per-size lists
+    const int32_t *pred = nullptr;  // set while enqueuing a device-driven step: RHS launches run while *pred != 0
 };
 
 

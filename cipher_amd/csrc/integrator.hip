@@ -18,11 +18,19 @@
 //
 // An attempt is a fixed sequence of launches whose only varying input is dt,
 // kept in device memory, so it can be captured once in a HIP graph and
-// replayed; the only host synchronisation per attempt is the error norm.
+// replayed.  cipher_ts_solve runs whole steps on the device: a one-thread
+// kernel starts each attempt (step size with the exact final time, or no
+// work once the loop has ended), the attempt's launches are predicated on
+// it, a second one-thread kernel takes the controller's decision from the
+// error norm (ctrl.h: the same IEEE arithmetic as cipher_ts_adapt), and the
+// copy and PostStep of an accepted step are predicated on that decision; the
+// host synchronises once per group of attempts.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <string>
+
+#include <algorithm>
 
 #include "ctx.h"
 
@@ -53,6 +61,13 @@ void cipher_ts_free(cipher_ctx *c) {
     s.Y = s.Xn = s.Xh = s.dt_dev = nullptr;
     if (s.h_dt) (void)hipHostFree(s.h_dt);
     s.h_dt = nullptr;
+    if (s.dev) (void)hipFree(s.dev);
+    if (s.live) (void)hipFree(s.live);
+    if (s.acc) (void)hipFree(s.acc);
+    if (s.h_dev) (void)hipHostFree(s.h_dev);
+    s.dev = nullptr;
+    s.h_dev = nullptr;
+    s.live = s.acc = nullptr;
     if (s.own) (void)hipStreamDestroy(s.own);
     s.own = nullptr;
     s.n = 0;
@@ -66,47 +81,18 @@ void cipher_ts_invalidate(cipher_ctx *c) {
     s.exec = nullptr;
     s.graph = nullptr;
     s.graph_X = nullptr;
+    if (s.sexec) (void)hipGraphExecDestroy(s.sexec);
+    if (s.sgraph) (void)hipGraphDestroy(s.sgraph);
+    s.sexec = nullptr;
+    s.sgraph = nullptr;
+    s.sgraph_X = nullptr;
 }
 
 extern "C" int cipher_ts_adapt(const cipher_ts_params *p, cipher_ts_ctrl *st, double dt, double err, int *accept,
                                double *dt_next) {
     if (!p || !st || !accept || !dt_next || !(dt > 0.0)) return cipher_fail(CIPHER_ERR_ARG, "bad argument");
-    const bool at_min = dt <= (1.0 + 1.4901161193847656e-08) * p->dt_min;
-    double safety = st->safety > 0.0 ? st->safety : p->safety;
-    if (!std::isfinite(err)) {
-        // a stage blew up: shrink as far as allowed, fail at the minimum
-        *accept = 0;
-        st->rejected_last = 1;
-        *dt_next = std::fmax(dt * p->clip_lo, p->dt_min);
-        if (at_min) return cipher_fail(CIPHER_ERR_NONFINITE, "error norm is not finite at the minimum step");
-        return CIPHER_OK;
-    }
-    const bool acc = err <= 1.0 || at_min;
-    if (!acc && st->rejected_last) safety *= p->reject_safety;
-    // PI controller (Soederlind's PI.4.2: k*beta1 = 3/5, k*beta2 = -1/5,
-    // k = 3 for the second-order error estimate), elementary controller
-    // (exponent 1/k) without history, arctan limiter rho = 1 + atan(r - 1)
-    const double k = 3.0;
-    double rho;
-    if (err > 0.0) {
-        double r = std::pow(1.0 / err, (st->err_prev > 0.0 ? 0.6 : 1.0) / k);
-        if (st->err_prev > 0.0) r *= std::pow(1.0 / st->err_prev, -0.2 / k);
-        rho = 1.0 + std::atan(r - 1.0);
-    } else {
-        rho = p->clip_hi;
-    }
-    double ratio = safety * rho;
-    ratio = std::fmin(std::fmax(ratio, p->clip_lo), p->clip_hi);
-    *dt_next = std::fmin(std::fmax(dt * ratio, p->dt_min), p->dt_max);
-    *accept = acc ? 1 : 0;
-    if (acc) {
-        st->err_prev = err;
-        st->safety = p->safety;
-        st->rejected_last = 0;
-    } else {
-        st->safety = safety;
-        st->rejected_last = 1;
-    }
+    if (ctl_adapt(*p, *st, dt, err, accept, dt_next) != CIPHER_OK)
+        return cipher_fail(CIPHER_ERR_NONFINITE, "error norm is not finite at the minimum step");
     return CIPHER_OK;
 }
 
@@ -129,6 +115,10 @@ extern "C" int cipher_ts_setup(cipher_ctx *c, const cipher_ts_params *p) {
     TSCHK(hipMalloc(&s.Xh, bytes));
     TSCHK(hipMalloc(&s.dt_dev, sizeof(double)));
     TSCHK(hipHostMalloc(&s.h_dt, sizeof(double), hipHostMallocDefault));
+    TSCHK(hipMalloc(&s.dev, sizeof(TsDev)));
+    TSCHK(hipHostMalloc(&s.h_dev, sizeof(TsDev), hipHostMallocDefault));
+    TSCHK(hipMalloc(&s.live, sizeof(int32_t)));
+    TSCHK(hipMalloc(&s.acc, sizeof(int32_t)));
     s.ready = true;
     return CIPHER_OK;
 }
@@ -173,6 +163,77 @@ static int capture(cipher_ctx *c, const double *X) {
     s.graph = g;
     TSCHK(hipGraphInstantiate(&s.exec, g, nullptr, nullptr, 0));
     s.graph_X = X;
+    return CIPHER_OK;
+}
+
+// ---- device-driven steps ---------------------------------------------------
+// start of an attempt (one thread): the step size and the attempt's run flag
+__global__ void k_ts_begin(TsDev *s, double *dt_dev, const int32_t *post_err, int32_t *live) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ts_begin(*s, *post_err, dt_dev, live);
+}
+
+// end of an attempt (one thread): the controller's decision from the norm
+__global__ void k_ts_decide(TsDev *s, const double *norm, cipher_ts_params p, int32_t *acc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ts_decide(*s, p, *norm, acc);
+}
+
+// X = Xn where the attempt was accepted
+__global__ __launch_bounds__(256) void k_copy_if(long long n, double *X, const double *Xn, const int32_t *acc) {
+    if (*acc == 0) return;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        X[i] = Xn[i];
+}
+
+// one step on the device: begin, the attempt (predicated on its run flag),
+// the decision, and the copy and PostStep of an accepted step (predicated on
+// the decision); a capacity overflow of PostStep ends the loop at the next
+// begin
+static int enqueue_step(cipher_ctx *c, double *X) {
+    TsState &s = c->ts;
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(k_ts_begin, dim3(1), dim3(64), 0, st, s.dev, s.dt_dev, (const int32_t *)c->err, s.live);
+    c->pred = s.live;
+    int rc = enqueue_attempt(c, X);
+    c->pred = nullptr;
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_ts_decide, dim3(1), dim3(64), 0, st, s.dev, (const double *)c->norm, s.p, s.acc);
+    long long blocks = (s.n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_copy_if, dim3((unsigned)blocks), dim3(256), 0, st, s.n, X, (const double *)s.Xn,
+                       (const int32_t *)s.acc);
+    PostLaunch L;
+    L.cfg = &c->dc;
+    L.X = X;
+    L.cell_off = c->cell_off;
+    L.adj = c->adj;
+    L.glist = c->glist;
+    L.err = c->err;
+    L.phase = nullptr;
+    L.ncells = c->ncells;
+    L.pred = s.acc;
+    if (launch_poststep(L, st) != CIPHER_OK) return cipher_fail(CIPHER_ERR_HIP, "poststep launch");
+    return hipGetLastError() == hipSuccess ? CIPHER_OK : cipher_fail(CIPHER_ERR_HIP, "step launch");
+}
+
+static int capture_step(cipher_ctx *c, double *X) {
+    TsState &s = c->ts;
+    if (s.sexec) (void)hipGraphExecDestroy(s.sexec);
+    if (s.sgraph) (void)hipGraphDestroy(s.sgraph);
+    s.sexec = nullptr;
+    s.sgraph = nullptr;
+    TSCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    int rc = enqueue_step(c, X);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return cipher_fail(CIPHER_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    s.sgraph = g;
+    TSCHK(hipGraphInstantiate(&s.sexec, g, nullptr, nullptr, 0));
+    s.sgraph_X = X;
     return CIPHER_OK;
 }
 
@@ -223,34 +284,70 @@ extern "C" int cipher_ts_solve(cipher_ctx *c, double *X, double t0, double t_end
     if (!c || !X || !dt_inout || !stats || !(t_end >= t0)) return cipher_fail(CIPHER_ERR_ARG, "bad argument");
     TsState &s = c->ts;
     if (!s.ready) return cipher_fail(CIPHER_ERR_STATE, "cipher_ts_setup not called for this mesh");
-    cipher_ts_stats out{};
-    double t = t0;
-    double dt = *dt_inout > 0.0 ? *dt_inout : s.p.dt_min;
-    int rc = CIPHER_OK;
-    while (t < t_end && (max_steps <= 0 || out.steps < max_steps)) {
-        // exact final time: the last step ends at t_end
-        const double dt_try = (t + dt > t_end) ? t_end - t : dt;
-        if (!(dt_try > 0.0)) break;
-        double err = 0.0;
-        if ((rc = cipher_ts_attempt(c, X, dt_try, &err))) break;
-        out.rhs_evals += 4;
-        out.err = err;
-        int acc = 0;
-        double dt_next = dt_try;
-        if ((rc = cipher_ts_adapt(&s.p, &s.ctrl, dt_try, err, &acc, &dt_next))) break;
-        if (acc) {
-            if ((rc = cipher_ts_accept(c, X))) break;
-            t = (dt_try == t_end - t) ? t_end : t + dt_try;
-            out.steps++;
-        } else {
-            out.rejected++;
-        }
-        dt = dt_next;
+    bool restore = false;
+    if (s.p.use_graph && !c->stream) {
+        // the legacy default stream cannot be captured: use a stream of our
+        // own that synchronises with it
+        if (!s.own) TSCHK(hipStreamCreate(&s.own));
+        c->stream = s.own;
+        restore = true;
     }
-    out.t = t;
-    out.dt = dt;
+    TsDev &h = *s.h_dev;
+    h = TsDev{};
+    h.t = t0;
+    h.t_end = t_end;
+    h.dt = *dt_inout > 0.0 ? *dt_inout : s.p.dt_min;
+    h.ctrl = s.ctrl;
+    h.max_steps = max_steps;
+    h.empty = s.n > 0 ? 0 : 1;
+    int rc = CIPHER_OK;
+    int64_t syncs = 0;
+    hipError_t e = hipMemcpyAsync(s.dev, s.h_dev, sizeof(TsDev), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->err, 0, sizeof(int32_t), c->stream);
+    if (e != hipSuccess) rc = cipher_fail(CIPHER_ERR_HIP, "solve set-up");
+    int group = 1;
+    while (!rc) {
+        for (int k = 0; k < group && !rc; k++) {
+            if (s.p.use_graph) {
+                if (!s.sexec || s.sgraph_X != X) rc = capture_step(c, X);
+                if (!rc && hipGraphLaunch(s.sexec, c->stream) != hipSuccess)
+                    rc = cipher_fail(CIPHER_ERR_HIP, "graph launch");
+            } else {
+                rc = enqueue_step(c, X);
+            }
+        }
+        if (rc) break;
+        e = hipMemcpyAsync(s.h_dev, s.dev, sizeof(TsDev), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        syncs++;
+        if (e != hipSuccess) {
+            rc = cipher_fail(CIPHER_ERR_HIP, std::string("step: ") + hipGetErrorString(e));
+            break;
+        }
+        if (h.done) break;
+        // the next group: the attempts that reach t_end at the current step
+        // size (one more for a rejection), at most twice the last group and 16
+        const double rem = (h.t_end - h.t) / h.dt;
+        int want = rem < 15.0 ? (int)rem + 2 : 16;
+        if (h.max_steps > 0 && h.max_steps - h.steps + 1 < want) want = (int)(h.max_steps - h.steps + 1);
+        group = std::min(std::min(2 * group, 16), std::max(want, 1));
+    }
+    if (!rc && h.status == CIPHER_ERR_CAPACITY)
+        rc = cipher_fail(CIPHER_ERR_CAPACITY, "active-set union exceeds the capacity PF_SIZE");
+    if (!rc && h.status == CIPHER_ERR_NONFINITE)
+        rc = cipher_fail(CIPHER_ERR_NONFINITE, "error norm is not finite at the minimum step");
+    s.ctrl = h.ctrl;
+    cipher_ts_stats out{};
+    out.steps = h.steps;
+    out.rejected = h.rejected;
+    out.rhs_evals = h.rhs_evals;
+    out.t = h.t;
+    out.dt = h.dt;
+    out.err = h.err;
+    out.host_syncs = syncs;
     *stats = out;
-    *dt_inout = dt;
+    *dt_inout = h.dt;
+    if (restore) c->stream = nullptr;
     return rc;
 }
 

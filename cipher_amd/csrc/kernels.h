@@ -23,6 +23,7 @@ struct RhsArgs {
     long long ncells;
     int32_t map;              // block order: 0 as launched, 1 one contiguous range per XCD
     int32_t pad_;
+    const int32_t *pred;      // non-null: the launch does nothing while *pred == 0
 };
 
 // Launch shape of one per-size cell list: lanes per cell (at least the
@@ -81,6 +82,7 @@ struct RhsLaunch {
     // 0 or 1: bin the cells; 2 or 3: reuse the lists of the previous call,
     // whose state had the same active sets (stages of one attempt)
     int bins_mode;
+    const int32_t *pred = nullptr;  // device flag: the launches do nothing while it is 0
 };
 
 cipher_status launch_rhs(const RhsLaunch &L, int precision, hipStream_t st);
@@ -105,6 +107,7 @@ struct PostLaunch {
     int32_t *err;             // device flag (0 ok, 1 capacity)
     const int32_t *phase;     // setup only
     long long ncells;
+    const int32_t *pred = nullptr;  // device flag: the update does nothing while it is 0
 };
 cipher_status launch_poststep(const PostLaunch &L, hipStream_t st);
 cipher_status launch_setup(const PostLaunch &L, hipStream_t st);

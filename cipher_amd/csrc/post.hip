@@ -128,7 +128,9 @@ __device__ __forceinline__ void project_cells(double *X, int32_t *glist, long lo
 }
 
 template <int G>
-__global__ __launch_bounds__(256) void k_project(double *X, int32_t *glist, long long ncells, DevCfg c) {
+__global__ __launch_bounds__(256) void k_project(double *X, int32_t *glist, long long ncells, DevCfg c,
+                                                  const int32_t *pred) {
+    if (pred && *pred == 0) return;
     project_cells<G>(X, glist, ncells, c, blockIdx.x);
 }
 
@@ -203,7 +205,9 @@ __device__ __forceinline__ void union_cells(double *X, const int32_t *glist, con
 
 template <int G, int NC>
 __global__ __launch_bounds__(256) void k_union(double *X, const int32_t *glist, const int32_t *cell_off,
-                                               const int32_t *adj, int32_t *err, long long ncells, DevCfg c) {
+                                               const int32_t *adj, int32_t *err, long long ncells, DevCfg c,
+                                               const int32_t *pred) {
+    if (pred && *pred == 0) return;
     union_cells<G, NC>(X, glist, cell_off, adj, err, ncells, c, blockIdx.x);
 }
 
@@ -310,9 +314,10 @@ static cipher_status post_G(const PostLaunch &L, hipStream_t st, bool setup) {
                                L.cell_off, L.adj, L.err, L.ncells, c);                                       \
         else                                                                                                 \
             hipLaunchKernelGGL((k_union<G, NC>), dim3((unsigned)nb), dim3(256), 0, st, L.X, L.glist,         \
-                               L.cell_off, L.adj, L.err, L.ncells, c);                                       \
+                               L.cell_off, L.adj, L.err, L.ncells, c, L.pred);                               \
         break;
-    if (!setup) hipLaunchKernelGGL((k_project<G>), dim3((unsigned)nb), dim3(256), 0, st, L.X, L.glist, L.ncells, c);
+    if (!setup)
+        hipLaunchKernelGGL((k_project<G>), dim3((unsigned)nb), dim3(256), 0, st, L.X, L.glist, L.ncells, c, L.pred);
     switch (c.ncp) {
         CIPHER_POST_NC(1)
         CIPHER_POST_NC(2)

@@ -1247,6 +1247,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
 template <int GMAX, int NC, class R>
 __global__ __launch_bounds__(256, NC == 1 ? CIPHER_RHS_MINB1 : CIPHER_RHS_MINBN) void k_rhs_all(RhsArgs a, DevCfg c, const int32_t *bins,
                                                                    int32_t *redo, int32_t *counts, ListRanges rg) {
+    if (a.pred && *a.pred == 0) return;
     int r = 0;
     while (r + 1 < rg.n && (int)blockIdx.x >= rg.start[r + 1]) r++;
     const int q = rg.list[r];
@@ -1272,7 +1273,8 @@ __global__ __launch_bounds__(256, NC == 1 ? CIPHER_RHS_MINB1 : CIPHER_RHS_MINBN)
 template <int GMAX, int NC>
 __global__ __launch_bounds__(256) void k_prep_all(const double *__restrict__ X, double *__restrict__ cm,
                                                   long long ncells, DevCfg c, const int32_t *bins,
-                                                  const int32_t *counts, ListRanges rg) {
+                                                  const int32_t *counts, ListRanges rg, const int32_t *pred) {
+    if (pred && *pred == 0) return;
     int r = 0;
     while (r + 1 < rg.n && (int)blockIdx.x >= rg.start[r + 1]) r++;
     const int q = rg.list[r];
@@ -1360,7 +1362,8 @@ __device__ __forceinline__ void bin_tile(const double *__restrict__ X, long long
 template <int U>
 __global__ __launch_bounds__(256) void k_bin(const double *__restrict__ X, long long ncells, DevCfg c, int nbins,
                                               int32_t *__restrict__ bins, long long binstride,
-                                              int32_t *__restrict__ counts) {
+                                              int32_t *__restrict__ counts, const int32_t *pred) {
+    if (pred && *pred == 0) return;
     bin_tile(X, ncells, c, nbins, bins, binstride, counts, blockIdx.x);
 }
 
@@ -1415,6 +1418,7 @@ cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st) {
     a.cm = (const double *)L.cm; a.ncells = N;
     a.cells = nullptr; a.cells_count = nullptr;
     a.redo = nullptr; a.redo_count = nullptr; a.map = 0; a.pad_ = 0;
+    a.pred = L.pred;
     // counts[0..4]: list sizes, counts[8..12]: their FP32-mixed recomputation
     // lists.  A call that bins finds them cleared (L.fill covers them: this
     // call's own set, or a whole ring of sets cleared once for many calls);
@@ -1426,7 +1430,7 @@ cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st) {
             hipLaunchKernelGGL(k_fill_i32<0>, dim3((unsigned)((L.nfill + 255) / 256)), dim3(256), 0, st, L.fill,
                                L.nfill, 0);
         hipLaunchKernelGGL(k_bin<0>, dim3((unsigned)((N + BIN_TILE - 1) / BIN_TILE)), dim3(256), 0, st, L.X, N, c,
-                           nbins, L.bins, N, L.counts);
+                           nbins, L.bins, N, L.counts, L.pred);
     } else if (precision == CIPHER_FP32_MIXED) {
         hipLaunchKernelGGL(k_fill_i32<0>, dim3(1), dim3(64), 0, st, L.counts + 8, RHS_COUNTERS - 8, 0);
     }
@@ -1442,7 +1446,7 @@ cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st) {
         long long tp = 0;
         const ListRanges pr = make_ranges(ps, N, c, nbins, &l0, &tp);
         hipLaunchKernelGGL((k_prep_all<G, NC>), dim3((unsigned)tp), dim3(256), 0, st, L.X, (double *)L.cm, N, c,
-                           (const int32_t *)L.bins, (const int32_t *)L.counts, pr);
+                           (const int32_t *)L.bins, (const int32_t *)L.counts, pr, L.pred);
     }
     size_t lds = 0;
     long long tot = 0;

@@ -167,6 +167,9 @@ template <int NK>
 __global__ __launch_bounds__(256) void k_axpy_dt(long long n, double *__restrict__ Y, const double *__restrict__ X,
                                                  const double *__restrict__ dtp, AxpyArgs p) {
     const double dt = dtp[0];
+    // a zero step is an attempt that does not run (the device-driven loop
+    // has ended); the integrator never takes one otherwise
+    if (dt == 0.0) return;
     double a[NK];
 #pragma unroll
     for (int j = 0; j < NK; j++) a[j] = dt * p.a[j];

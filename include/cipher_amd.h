@@ -303,12 +303,15 @@ typedef struct cipher_ts_stats {
     double t;                   /* time reached */
     double dt;                  /* next step size */
     double err;                 /* error norm of the last attempt */
+    int64_t host_syncs;         /* times the host waited for the device */
 } cipher_ts_stats;
 
 /* Host-only step-size decision for an attempt of size dt with error norm err
  * (no device work): *accept = err <= 1 or dt at dt_min; *dt_next from the
  * PI controller with an arctan limiter, clipped and clamped to the limits.
- * CIPHER_ERR_NONFINITE for a non-finite err at the minimum step. */
+ * CIPHER_ERR_NONFINITE for a non-finite err at the minimum step.  The
+ * device-driven solves take the same decisions on the device, in the same
+ * IEEE operations (identical bits). */
 int cipher_ts_adapt(const cipher_ts_params *p, cipher_ts_ctrl *st, double dt, double err,
                     int *accept, double *dt_next);
 
@@ -324,7 +327,12 @@ int cipher_ts_attempt(cipher_ctx *ctx, const double *X_dev, double dt, double *e
 int cipher_ts_accept(cipher_ctx *ctx, double *X_dev);
 
 /* Integrate X_dev from t0 to t_end (exact final time) or for max_steps
- * accepted steps; *dt_inout: first step size in, next step size out. */
+ * accepted steps; *dt_inout: first step size in, next step size out.  The
+ * step decisions run on the device (after each attempt a one-thread kernel
+ * accepts or rejects it, updates the time and the step size; the copy and
+ * PostStep of an accepted step are predicated on its decision), so the
+ * host enqueues several attempts per synchronisation (up to 16, fewer
+ * near the end time); attempts enqueued after the end do no work. */
 int cipher_ts_solve(cipher_ctx *ctx, double *X_dev, double t0, double t_end, double *dt_inout,
                     int64_t max_steps, cipher_ts_stats *stats);
 

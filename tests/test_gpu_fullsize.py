@@ -63,6 +63,20 @@ def _rhs(ctx, X):
     return _host(F), _host(act).astype(np.uint32)
 
 
+def _rms(F, Fref, L):
+    """Per field: RMS of F - Fref over max|Fref| (normwise)."""
+    D = L["ndof"]
+    A, B = F.reshape(-1, D), Fref.reshape(-1, D)
+    out = {}
+    for name, sl in H.field_slices(L).items():
+        a, b = A[:, sl], B[:, sl]
+        if a.size:
+            s = np.abs(b).max()
+            r = float(np.sqrt(np.mean((a - b) ** 2)))
+            out[name] = r / s if s > 0 else r
+    return out
+
+
 def _check64(label, wl, X, Fg, ag, Fo, ao, exact=False):
     """FP64 against the oracle; returns the oracle's one-ulp spread."""
     assert np.array_equal(ag, ao), f"{label}: active flags differ in {np.count_nonzero(ag != ao)} cells"
@@ -72,7 +86,8 @@ def _check64(label, wl, X, Fg, ag, Fo, ao, exact=False):
         return None
     errs = H.field_errors(Fg, Fo, wl.layout)
     spread = H.ulp_spread(wl, X, Fo, trials=1)
-    print(f"   {label}: FP64 errors {errs}, oracle one-ulp spread {spread}", flush=True)
+    print(f"   {label}: FP64 errors max {errs} rms {_rms(Fg, Fo, wl.layout)}, oracle one-ulp spread {spread}",
+          flush=True)
     for f, e in errs.items():
         bar = max(RTOL64, 4.0 * spread[f])
         assert e <= bar, f"{label} field {f}: rel err {e:.3e} > {bar:.3e}"
@@ -89,7 +104,7 @@ def _check32(label, wl, F32, a32, Fo, ao, spread=None):
     FP64 value is rounding noise and its spread is of order 1)."""
     assert np.array_equal(a32, ao), f"{label}: FP32-mixed active flags differ in {np.count_nonzero(a32 != ao)} cells"
     errs = H.field_errors(F32, Fo, wl.layout)
-    print(f"   {label}: FP32-mixed errors vs oracle {errs}", flush=True)
+    print(f"   {label}: FP32-mixed errors vs oracle max {errs} rms {_rms(F32, Fo, wl.layout)}", flush=True)
     assert errs["PF"] <= RTOL32_PF, f"{label}: FP32-mixed phase-fraction rate rel err {errs['PF']:.3e}"
     if spread is not None:
         for f, e in errs.items():

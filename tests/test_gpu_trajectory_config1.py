@@ -49,6 +49,9 @@ def test_config1_trajectory_matches_cpu():
     assert (st.steps, st.rejected) == (info["steps"], info["rejected"])
     assert st.t == info["t"] == T_END
     assert dt == pytest.approx(info["dt"], rel=1e-8, abs=0.0)
+    attempts = st.steps + st.rejected
+    print(f"   {attempts} attempts, {st.host_syncs} host syncs", flush=True)
+    assert st.host_syncs <= attempts // 8 + 8
     L = wl.layout
     m = H.state_mask(Xc, L, wl.cfg)
     assert np.array_equal(H.state_mask(Xg, L, wl.cfg), m), "active sets differ at t_end"
