@@ -77,9 +77,10 @@ __device__ __forceinline__ R dfastpow(R base, unsigned e) {
     return r;
 }
 
-// SumTSeries, src/utility.c:48-62
+// SumTSeries, src/utility.c:48-62 (out of line: cells at the folding
+// temperature never call it, and one copy serves every call site)
 template <class R>
-__device__ inline R dtseries(const cipher_tseries *t, R T) {
+__device__ __noinline__ R dtseries(const cipher_tseries *t, R T) {
     R r = R(0);
     const int n = t->n;
     for (int i = 0; i < n; i++) {
@@ -95,16 +96,22 @@ __device__ inline R dtseries(const cipher_tseries *t, R T) {
 }
 
 // SumTSeries of table entry i; a cell at the folding temperature reads the
-// value the fold kernel computed with the same code (identical bits)
-template <class R>
+// value the fold kernel computed with the same code (identical bits).  ISO:
+// the caller has checked that every lane of the wave is at the folding
+// temperature, so the table value is returned without a per-lane test.
+template <class R, bool ISO = false>
 __device__ __forceinline__ R dts(const DevCfg &c, int i, R T) {
-    if (c.tsv && (double)T == c.tsv[c.ntseries]) return (R)c.tsv[i];
-    return dtseries<R>(&c.ts[i], T);
+    if constexpr (ISO) {
+        return (R)c.tsv[i];
+    } else {
+        if (c.tsv && (double)T == c.tsv[c.ntseries]) return (R)c.tsv[i];
+        return dtseries<R>(&c.ts[i], T);
+    }
 }
 
 // SumTSeries_derivative, src/utility.c:67-79
 template <class R>
-__device__ inline R dtseries_deriv(const cipher_tseries *t, R T) {
+__device__ __noinline__ R dtseries_deriv(const cipher_tseries *t, R T) {
     R r = R(0);
     for (int i = 0; i < t->n; i++) {
         const int e = t->exp[i];
@@ -227,7 +234,7 @@ __device__ __forceinline__ void daddat(R *a, int ix, R v) {
 // potentials; y: nsites*NC site fractions.  The closed form is evaluated in
 // FP64 in both precisions (exp, normalisation and the 1-sum remainder are
 // the end-member sensitive part).
-template <int NC, class R>
+template <int NC, class R, bool ISO = false>
 __device__ inline void d_sitefrac(const DevCfg &c, const cipher_material &m, R *y, const R *spim, R T) {
     constexpr int ND = NC - 1;
     if (m.model == CIPHER_CHEM_CALPHADDIS || m.model == CIPHER_CHEM_CALPHAD2SL) {
@@ -260,9 +267,9 @@ __device__ inline void d_sitefrac(const DevCfg &c, const cipher_material &m, R *
         R mt[Cnt<ND * ND>::v], mi[Cnt<ND * ND>::v];
 #pragma unroll
         for (int k = 0; k < NC; k++) {
-            ceq[k] = dts<R>(c, m.quad_ceq + k, T);
-            un[k] = dts<R>(c, m.unary + k, T);
-            bi[k] = dts<R>(c, m.quad_binary + k, T);
+            ceq[k] = dts<R, ISO>(c, m.quad_ceq + k, T);
+            un[k] = dts<R, ISO>(c, m.unary + k, T);
+            bi[k] = dts<R, ISO>(c, m.quad_binary + k, T);
         }
 #pragma unroll
         for (int k = 0; k < ND * ND; k++) { mt[k] = R(0); mi[k] = R(0); }
@@ -287,7 +294,7 @@ __device__ inline void d_sitefrac(const DevCfg &c, const cipher_material &m, R *
 }
 
 // CompositionMobility{,Component}, src/material.c:1023-1120 (times Vm)
-template <int NC, class R>
+template <int NC, class R, bool ISO = false>
 __device__ inline void d_mobility(const DevCfg &c, const cipher_material &m, R *mob, const R *y, R T) {
     if (m.model == CIPHER_CHEM_NONE) {
 #pragma unroll
@@ -301,13 +308,13 @@ __device__ inline void d_mobility(const DevCfg &c, const cipher_material &m, R *
             R mig = R(0);
 #pragma unroll
             for (int j = 0; j < NC; j++) {
-                mig += y[j] * dts<R>(c, mb.unary + j, T);
+                mig += y[j] * dts<R, ISO>(c, mb.unary + j, T);
 #pragma unroll
                 for (int i = j + 1; i < NC; i++, r++) {
                     if (!((nz >> r) & 1)) continue;
                     const cipher_rk bin = c.rk[mb.binary + r];
                     for (int o = 0; o < bin.n; o++)
-                        mig += y[j] * y[i] * dts<R>(c, bin.ts + o, T) * dfastpow(y[j] - y[i], (unsigned)o);
+                        mig += y[j] * y[i] * dts<R, ISO>(c, bin.ts + o, T) * dfastpow(y[j] - y[i], (unsigned)o);
                 }
             }
             mob[k] = y[k] * (R)mb.m0 / (R)CRG / T * dexp<R>(mig / (R)CRG / T);
@@ -318,7 +325,7 @@ __device__ inline void d_mobility(const DevCfg &c, const cipher_material &m, R *
 }
 
 // Chemenergy_{calphad2sl,calphaddis,quad,none}, src/material.c:356-475
-template <int NC, class R>
+template <int NC, class R, bool ISO = false>
 __device__ inline R d_chemenergy(const DevCfg &c, const cipher_material &m, const R *y, R T) {
     if (m.model == CIPHER_CHEM_CALPHAD2SL) {
         const R *yp = y, *yq = y + NC;
@@ -329,7 +336,7 @@ __device__ inline R d_chemenergy(const DevCfg &c, const cipher_material &m, cons
 #pragma unroll
         for (int cp = 0; cp < NC; cp++)
 #pragma unroll
-            for (int cq = 0; cq < NC; cq++) e += dts<R>(c, m.unary + cp * NC + cq, T) * yp[cp] * yq[cq];
+            for (int cq = 0; cq < NC; cq++) e += dts<R, ISO>(c, m.unary + cp * NC + cq, T) * yp[cp] * yq[cq];
         {
             const uint64_t nz = rk_window(c, m.binary);
             int r = 0;
@@ -342,7 +349,7 @@ __device__ inline R d_chemenergy(const DevCfg &c, const cipher_material &m, cons
                         if (!((nz >> r) & 1)) continue;
                         const cipher_rk rv = c.rk[m.binary + r];
                         for (int o = 0; o < rv.n; o++)
-                            e += dts<R>(c, rv.ts + o, T) * yp[i] * yp[j] * yq[cq] * dfastpow(yp[i] - yp[j], (unsigned)o);
+                            e += dts<R, ISO>(c, rv.ts + o, T) * yp[i] * yp[j] * yq[cq] * dfastpow(yp[i] - yp[j], (unsigned)o);
                     }
         }
         {
@@ -357,7 +364,7 @@ __device__ inline R d_chemenergy(const DevCfg &c, const cipher_material &m, cons
                         if (!((nz >> r) & 1)) continue;
                         const cipher_rk rv = c.rk[m.binaryq + r];
                         for (int o = 0; o < rv.n; o++)
-                            e += dts<R>(c, rv.ts + o, T) * yp[cp] * yq[i] * yq[j] * dfastpow(yq[i] - yq[j], (unsigned)o);
+                            e += dts<R, ISO>(c, rv.ts + o, T) * yp[cp] * yq[i] * yq[j] * dfastpow(yq[i] - yq[j], (unsigned)o);
                     }
         }
         {
@@ -374,7 +381,7 @@ __device__ inline R d_chemenergy(const DevCfg &c, const cipher_material &m, cons
                             if (!((nz >> r) & 1)) continue;
                             const cipher_rk rv = c.rk[m.ternary + r];
                             for (int o = 0; o < rv.n; o++)
-                                e += dts<R>(c, rv.ts + o, T) * yp[i] * yp[j] * yp[k] * yq[cq];
+                                e += dts<R, ISO>(c, rv.ts + o, T) * yp[i] * yp[j] * yp[k] * yq[cq];
                         }
         }
         {
@@ -391,50 +398,50 @@ __device__ inline R d_chemenergy(const DevCfg &c, const cipher_material &m, cons
                             if (!((nz >> r) & 1)) continue;
                             const cipher_rk rv = c.rk[m.ternaryq + r];
                             for (int o = 0; o < rv.n; o++)
-                                e += dts<R>(c, rv.ts + o, T) * yq[i] * yq[j] * yq[k] * yp[cp];
+                                e += dts<R, ISO>(c, rv.ts + o, T) * yq[i] * yq[j] * yq[k] * yp[cp];
                         }
         }
         return e;
     } else if (m.model == CIPHER_CHEM_CALPHADDIS) {
         const uint64_t nzb = rk_window(c, m.binary), nzt = rk_window(c, m.ternary);
         int rb = 0, rt = 0;
-        R e = dts<R>(c, m.ref, T);
+        R e = dts<R, ISO>(c, m.ref, T);
 #pragma unroll
         for (int k = 0; k < NC; k++) {
-            e += dts<R>(c, m.unary + k, T) * y[k] + (R)CRG * T * y[k] * dlog(y[k]);
+            e += dts<R, ISO>(c, m.unary + k, T) * y[k] + (R)CRG * T * y[k] * dlog(y[k]);
 #pragma unroll
             for (int j = k + 1; j < NC; j++, rb++) {
                 if ((nzb >> rb) & 1) {
                     const cipher_rk bin = c.rk[m.binary + rb];
                     for (int o = 0; o < bin.n; o++)
-                        e += dts<R>(c, bin.ts + o, T) * y[k] * y[j] * dfastpow(y[k] - y[j], (unsigned)o);
+                        e += dts<R, ISO>(c, bin.ts + o, T) * y[k] * y[j] * dfastpow(y[k] - y[j], (unsigned)o);
                 }
 #pragma unroll
                 for (int i = j + 1; i < NC; i++, rt++) {
                     if (!((nzt >> rt) & 1)) continue;
                     const cipher_rk ter = c.rk[m.ternary + rt];
                     for (int o = 0; o < ter.n; o++)
-                        e += dts<R>(c, ter.ts + o, T) * y[k] * y[j] * y[i]
+                        e += dts<R, ISO>(c, ter.ts + o, T) * y[k] * y[j] * y[i]
                              * (dsel<NC, R>(y, c.rkindex[ter.idx + o]) + (R(1) - y[k] - y[j] - y[i]) / R(3));
                 }
             }
         }
         return e;
     } else if (m.model == CIPHER_CHEM_QUADRATIC) {
-        R e = dts<R>(c, m.ref, T);
+        R e = dts<R, ISO>(c, m.ref, T);
 #pragma unroll
         for (int k = 0; k < NC; k++) {
-            const R ceq = dts<R>(c, m.quad_ceq + k, T);
-            e += dts<R>(c, m.unary + k, T) * (y[k] - ceq)
-                 + dts<R>(c, m.quad_binary + k, T) * (y[k] - ceq) * (y[k] - ceq);
+            const R ceq = dts<R, ISO>(c, m.quad_ceq + k, T);
+            e += dts<R, ISO>(c, m.unary + k, T) * (y[k] - ceq)
+                 + dts<R, ISO>(c, m.quad_binary + k, T) * (y[k] - ceq) * (y[k] - ceq);
         }
         return e;
     }
-    return dts<R>(c, m.ref, T);
+    return dts<R, ISO>(c, m.ref, T);
 }
 
 // SitepotentialExplicit, src/material.c:495-650: out nsites*(NC-1)
-template <int NC, class R>
+template <int NC, class R, bool ISO = false>
 __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material &m, R *sp, const R *y, R T) {
     constexpr int ND = NC - 1;
     R full[2 * NC];
@@ -448,7 +455,7 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
         for (int cp = 0; cp < NC; cp++)
 #pragma unroll
             for (int cq = 0; cq < NC; cq++) {
-                a = dts<R>(c, m.unary + cp * NC + cq, T);
+                a = dts<R, ISO>(c, m.unary + cp * NC + cq, T);
                 pp[cp] += a * yq[cq];
                 pq[cq] += a * yp[cp];
             }
@@ -464,7 +471,7 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
                         if (!((nz >> r) & 1)) continue;
                         const cipher_rk rv = c.rk[m.binary + r];
                         for (int o = 0; o < rv.n; o++) {
-                            a = dts<R>(c, rv.ts + o, T);
+                            a = dts<R, ISO>(c, rv.ts + o, T);
                             b = dfastpow(yp[i] - yp[j], (unsigned)o);
                             cc = (o == 0) ? R(0) : ((R)o) * dfastpow(yp[i] - yp[j], (unsigned)(o - 1));
                             pp[i] += a * yp[j] * yq[cq] * (b + cc * yp[i]);
@@ -485,7 +492,7 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
                         if (!((nz >> r) & 1)) continue;
                         const cipher_rk rv = c.rk[m.binaryq + r];
                         for (int o = 0; o < rv.n; o++) {
-                            a = dts<R>(c, rv.ts + o, T);
+                            a = dts<R, ISO>(c, rv.ts + o, T);
                             b = dfastpow(yq[i] - yq[j], (unsigned)o);
                             cc = (o == 0) ? R(0) : ((R)o) * dfastpow(yq[i] - yq[j], (unsigned)(o - 1));
                             pq[i] += a * yq[j] * yp[cp] * (b + cc * yq[i]);
@@ -508,7 +515,7 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
                             if (!((nz >> r) & 1)) continue;
                             const cipher_rk rv = c.rk[m.ternary + r];
                             for (int o = 0; o < rv.n; o++) {
-                                a = dts<R>(c, rv.ts + o, T);
+                                a = dts<R, ISO>(c, rv.ts + o, T);
                                 pp[i] += a * yp[j] * yp[k] * yq[cq];
                                 pp[j] += a * yp[k] * yp[i] * yq[cq];
                                 pp[k] += a * yp[i] * yp[j] * yq[cq];
@@ -530,7 +537,7 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
                             if (!((nz >> r) & 1)) continue;
                             const cipher_rk rv = c.rk[m.ternaryq + r];
                             for (int o = 0; o < rv.n; o++) {
-                                a = dts<R>(c, rv.ts + o, T);
+                                a = dts<R, ISO>(c, rv.ts + o, T);
                                 pq[i] += a * yq[j] * yq[k] * yp[cp];
                                 pq[j] += a * yq[k] * yq[i] * yp[cp];
                                 pq[k] += a * yq[i] * yq[j] * yp[cp];
@@ -541,7 +548,7 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
     } else if (m.model == CIPHER_CHEM_CALPHADDIS) {
         R a, b, cc;
 #pragma unroll
-        for (int i = 0; i < NC; i++) full[i] = dts<R>(c, m.unary + i, T);
+        for (int i = 0; i < NC; i++) full[i] = dts<R, ISO>(c, m.unary + i, T);
         {
             const uint64_t nz = rk_window(c, m.binary);
             int r = 0;
@@ -552,7 +559,7 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
                     if (!((nz >> r) & 1)) continue;
                     const cipher_rk rv = c.rk[m.binary + r];
                     for (int o = 0; o < rv.n; o++) {
-                        a = dts<R>(c, rv.ts + o, T);
+                        a = dts<R, ISO>(c, rv.ts + o, T);
                         b = dfastpow(y[i] - y[j], (unsigned)o);
                         cc = (o == 0) ? R(0) : ((R)o) * dfastpow(y[i] - y[j], (unsigned)(o - 1));
                         full[i] += a * y[j] * (b + cc * y[i]);
@@ -572,7 +579,7 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
                     const cipher_rk rv = c.rk[m.ternary + rt];
                     for (int o = 0; o < rv.n; o++) {
                         const int ix = c.rkindex[rv.idx + o];
-                        a = dts<R>(c, rv.ts + o, T);
+                        a = dts<R, ISO>(c, rv.ts + o, T);
                         b = dsel<NC, R>(y, ix) + (R(1) - y[k] - y[j] - y[i]) / R(3);
                         full[i] += a * y[j] * y[k] * (b - y[i] / R(3));
                         full[j] += a * y[k] * y[i] * (b - y[j] / R(3));
@@ -591,7 +598,7 @@ __device__ inline void d_sitepot_explicit(const DevCfg &c, const cipher_material
 }
 
 // SitepotentialImplicit, src/material.c:655-711
-template <int NC, class R>
+template <int NC, class R, bool ISO = false>
 __device__ inline void d_sitepot_implicit(const DevCfg &c, const cipher_material &m, R *sp, const R *y, R T) {
     constexpr int ND = NC - 1;
     R full[2 * NC];
@@ -609,8 +616,8 @@ __device__ inline void d_sitepot_implicit(const DevCfg &c, const cipher_material
     } else if (m.model == CIPHER_CHEM_QUADRATIC) {
 #pragma unroll
         for (int k = 0; k < NC; k++)
-            full[k] = dts<R>(c, m.unary + k, T)
-                      + R(2) * dts<R>(c, m.quad_binary + k, T) * (y[k] - dts<R>(c, m.quad_ceq + k, T));
+            full[k] = dts<R, ISO>(c, m.unary + k, T)
+                      + R(2) * dts<R, ISO>(c, m.quad_binary + k, T) * (y[k] - dts<R, ISO>(c, m.quad_ceq + k, T));
     }
 #pragma unroll
     for (int s = 0; s < 2; s++) {
@@ -623,7 +630,7 @@ __device__ inline void d_sitepot_implicit(const DevCfg &c, const cipher_material
 // SitefracTangent, src/material.c:827-1018: tpot (nsites*ND*ND) and tT
 // (nsites*ND), both times Vm.  The ex tangent is -tpot for the CALPHAD
 // models and 0 for quadratic/none (returned through *ex_sign).
-template <int NC, class R>
+template <int NC, class R, bool ISO = false>
 __device__ inline void d_sitefrac_tangent(const DevCfg &c, const cipher_material &m, R *tpot, R *tT, R &ex_sign,
                                           const R *y, const R *spim, R T) {
     constexpr int ND = NC - 1, NN = ND * ND;
@@ -669,8 +676,8 @@ __device__ inline void d_sitefrac_tangent(const DevCfg &c, const cipher_material
             ceqT[k] = dts_deriv<R>(c, m.quad_ceq + k, T);
             unT[k] = dts_deriv<R>(c, m.unary + k, T);
             biT[k] = dts_deriv<R>(c, m.quad_binary + k, T);
-            ceq[k] = dts<R>(c, m.quad_ceq + k, T);
-            bi[k] = dts<R>(c, m.quad_binary + k, T);
+            ceq[k] = dts<R, ISO>(c, m.quad_ceq + k, T);
+            bi[k] = dts<R, ISO>(c, m.quad_binary + k, T);
         }
 #pragma unroll
         for (int k = 0; k < NN; k++) { mt[k] = R(0); mi[k] = R(0); }

@@ -70,6 +70,16 @@ __device__ __forceinline__ R itf_mobility(const DevCfg &c, int it, R T) {
     return m;
 }
 
+// every active lane of the wave holds a cell at the folding temperature of
+// the T-series table (then the thermodynamics read folded values without a
+// per-lane test); the result is uniform over the wave
+__device__ __forceinline__ bool wave_iso(const DevCfg &c, double T) {
+    return c.tsv != nullptr && __all(T == c.tsv[c.ntseries]);
+}
+__device__ __forceinline__ double dts_w(const DevCfg &c, int i, double T, bool iso) {
+    return iso ? c.tsv[i] : dts<double>(c, i, T);
+}
+
 // Kahan-compensated accumulator in FP32; plain sum (reference order) in FP64
 template <class R>
 struct Acc {
@@ -141,7 +151,7 @@ __device__ __forceinline__ void interface_mu(const DevCfg &c, int n, int myid, R
 
 // implicit site potentials and site fractions of the slot's phase, in FP64
 // (src/cipher.c:170-189 and 584-609): spim = nu_s (mu - mu_int) - EX
-template <int G, int NC>
+template <int G, int NC, bool ISO>
 __device__ __forceinline__ void slot_sitefrac(const DevCfg &c, const cipher_material &m, bool on, int ns, int n,
                                               int myid, double h, const double *x, int slot, double T, double *spim,
                                               double *y) {
@@ -156,7 +166,7 @@ __device__ __forceinline__ void slot_sitefrac(const DevCfg &c, const cipher_mate
                                         : 0.0;
 #pragma unroll
     for (int k = 0; k < 2 * NC; k++) y[k] = 0.0;
-    if (on) d_sitefrac<NC, double>(c, m, y, spim, T);
+    if (on) d_sitefrac<NC, double, ISO>(c, m, y, spim, T);
 }
 
 // ---------------------------------------------------------------------------
@@ -180,6 +190,7 @@ __device__ __forceinline__ void prep_one(const double *x, long long cell, double
     const int myid = on ? clamp_id(x[c.aso + 1 + slot], c.npf) : 0;
     const double phi = on ? x[c.pfo + slot] : 0.0;
     const double T = x[c.tmo];
+    const bool iso = wave_iso(c, T);
     double hh, h;
     interpolant<G, double>(c, n, phi, hh, h);
     if (!on) h = 0.0;
@@ -190,9 +201,11 @@ __device__ __forceinline__ void prep_one(const double *x, long long cell, double
     if constexpr (NC > 1) {
         double spim[2 * Cnt<ND>::v], y[2 * NC], me[NC];
         const int ns = on ? m.nsites : 0;
-        slot_sitefrac<G, NC>(c, m, on, ns, n, myid, h, x, slot, T, spim, y);
+        if (iso) slot_sitefrac<G, NC, true>(c, m, on, ns, n, myid, h, x, slot, T, spim, y);
+        else slot_sitefrac<G, NC, false>(c, m, on, ns, n, myid, h, x, slot, T, spim, y);
         if (on) {
-            d_mobility<NC, double>(c, m, me, y, T);
+            if (iso) d_mobility<NC, double, true>(c, m, me, y, T);
+            else d_mobility<NC, double, false>(c, m, me, y, T);
         } else {
 #pragma unroll
             for (int k = 0; k < NC; k++) me[k] = 0.0;
@@ -232,7 +245,7 @@ __device__ __forceinline__ void prep_one(const double *x, long long cell, double
         }
     }
     if (c.has_tcond) {
-        const double tv = (on && c.ts[m.tconductivity].n) ? h * dts<double>(c, m.tconductivity, T) : 0.0;
+        const double tv = (on && c.ts[m.tconductivity].n) ? h * dts_w(c, m.tconductivity, T, iso) : 0.0;
         #pragma unroll 1
         for (int g = 0; g < n; g++) {
             const int idg = gsh<G>(myid, g);
@@ -519,6 +532,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
         const R phi = (R)phid;
         const double Td = x[c.tmo];
         const R T = (R)Td;
+        const bool iso = wave_iso(c, Td);
         const double hS = a.cellgeom[cell];
         const int dim = c.dim;
 
@@ -792,7 +806,8 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
 
         // site fractions of the slot's phase, FP64 (same arithmetic as k_prep)
         double yd[2 * NC], spimd[2 * NDV];
-        slot_sitefrac<G, NC>(c, m, on, ns, n, myid, hd, x, slot, Td, spimd, yd);
+        if (iso) slot_sitefrac<G, NC, true>(c, m, on, ns, n, myid, hd, x, slot, Td, spimd, yd);
+        else slot_sitefrac<G, NC, false>(c, m, on, ns, n, myid, hd, x, slot, Td, spimd, yd);
         // interface type of every active pair, row gk of the group's table
         {
             int row[G];
@@ -838,9 +853,9 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                         mv += dfastpow(dot, 4u) * c.aniso[f.m_aniso + c.dim * f.m_naniso + r];
                     }
                     ev *= f.e0;
-                    if (c.has_iact && c.ts[f.e_act].n) ev *= dexp<double>(dts<double>(c, f.e_act, Td) / CRG / Td);
+                    if (c.has_iact && c.ts[f.e_act].n) ev *= dexp<double>(dts_w(c, f.e_act, Td, iso) / CRG / Td);
                     mv *= f.m0;
-                    if (c.has_iact && c.ts[f.m_act].n) mv *= dexp<double>(dts<double>(c, f.m_act, Td) / CRG / Td);
+                    if (c.has_iact && c.ts[f.m_act].n) mv *= dexp<double>(dts_w(c, f.m_act, Td, iso) / CRG / Td);
                     pe_tab[slot * G + gj] = ev;
                     pm_tab[slot * G + gj] = mv;
                 }
@@ -908,7 +923,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
             // chemical driving force (src/cipher.c:499-522; Chemenergy, src/material.c:480-490), FP64
             double chs = 0.0;
             if (on) {
-                double ch = d_chemenergy<NC, double>(c, m, yd, Td);
+                double ch = iso ? d_chemenergy<NC, double, true>(c, m, yd, Td) : d_chemenergy<NC, double, false>(c, m, yd, Td);
                 ch /= m.molarvolume;
 #pragma unroll
                 for (int s = 0; s < 2; s++) {
@@ -1047,8 +1062,8 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
         R tmrhs = tlap.s;
         if (c.has_theat || c.has_trate) {
             R cpv = R(0);
-            const R cpt = (on && c.ts[m.specific_heat].n) ? h * (R)dts<double>(c, m.specific_heat, Td) : R(0);
-            const R lht = (on && c.ts[m.latent_heat].n) ? (R)pfrhs * (R)dts<double>(c, m.latent_heat, Td) : R(0);
+            const R cpt = (on && c.ts[m.specific_heat].n) ? h * (R)dts_w(c, m.specific_heat, Td, iso) : R(0);
+            const R lht = (on && c.ts[m.latent_heat].n) ? (R)pfrhs * (R)dts_w(c, m.latent_heat, Td, iso) : R(0);
             #pragma unroll 1
             for (int g = 0; g < n; g++) {
                 const int idg = gsh<G>(myid, g);
@@ -1071,7 +1086,8 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
             // explicit potential rate (src/cipher.c:572-581), FP64
             if (on) {
                 double spw[2 * NDV];
-                d_sitepot_explicit<NC, double>(c, m, spw, yd, Td);
+                if (iso) d_sitepot_explicit<NC, double, true>(c, m, spw, yd, Td);
+                else d_sitepot_explicit<NC, double, false>(c, m, spw, yd, Td);
 #pragma unroll
                 for (int s = 0; s < 2; s++) {
                     if (s >= ns) break;
@@ -1112,7 +1128,8 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
 #pragma unroll
             for (int k = 0; k < 2 * ND; k++) tT[k] = 0.0;
             if (on) {
-                d_sitefrac_tangent<NC, double>(c, m, tpot, tT, exs, yd, spimd, Td);
+                if (iso) d_sitefrac_tangent<NC, double, true>(c, m, tpot, tT, exs, yd, spimd, Td);
+                else d_sitefrac_tangent<NC, double, false>(c, m, tpot, tT, exs, yd, spimd, Td);
             }
             double dcdm[ND * ND];
 #pragma unroll
