@@ -36,11 +36,11 @@ int cipher_fail(int code, const std::string &msg) { return fail(code, msg); }
 static void free_mesh(cipher_ctx *c) {
     if (c->dc.gw) (void)hipFree((void *)c->dc.gw);
     c->dc.gw = nullptr;
-    void *ptrs[] = {c->cell_off, c->adj, c->bcell_off, c->bface_id, c->cellgeom, c->cm, c->glist, c->list, c->bins};
+    void *ptrs[] = {c->cell_off, c->adj, c->fgeo, c->bcell_off, c->bface_id, c->cellgeom, c->cm, c->glist, c->list, c->bins};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->cell_off = c->adj = c->bcell_off = c->bface_id = nullptr;
-    c->cellgeom = nullptr;
+    c->cellgeom = c->fgeo = nullptr;
     c->cm = nullptr;
     c->glist = nullptr;
     c->list = nullptr;
@@ -397,6 +397,7 @@ extern "C" int cipher_upload_mesh(cipher_ctx *c, const cipher_mesh *m) {
     size_t cmrec = prep_record_bytes(c->dc.ncp);
     HIPCHK(hipMalloc(&c->cell_off, sizeof(int32_t) * (N + 1)));
     HIPCHK(hipMalloc(&c->adj, sizeof(int32_t) * adj.size()));
+    HIPCHK(hipMalloc(&c->fgeo, sizeof(double) * adj.size()));
     HIPCHK(hipMalloc(&c->bcell_off, sizeof(int32_t) * (N + 1)));
     HIPCHK(hipMalloc(&c->bface_id, sizeof(int32_t) * bid.size()));
     HIPCHK(hipMalloc(&c->cellgeom, sizeof(double) * std::max<long long>(N, 1)));
@@ -406,6 +407,31 @@ extern "C" int cipher_upload_mesh(cipher_ctx *c, const cipher_mesh *m) {
     HIPCHK(hipMalloc(&c->bins, 5 * sizeof(int32_t) * std::max<long long>(N, 1)));
     HIPCHK(hipMemcpy(c->cell_off, off.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->adj, adj.data(), sizeof(int32_t) * adj.size(), hipMemcpyHostToDevice));
+    {
+        // flux factor of every face entry, 2 * (the smaller cell's v/h) / (hL + hR)
+        // with v = h^dim (src/cipher.c:248-262), by the same FP64 operations
+        // the reference evaluates per face; the same for both cells of a face
+        std::vector<double> fg(adj.size(), 0.0);
+        const unsigned dim = (unsigned)c->dc.dim;
+        auto vol = [dim](double b) {
+            double r = 1.0;
+            for (unsigned e = dim; e; e >>= 1) {
+                if (e & 1u) r *= b;
+                b *= b;
+            }
+            return r;
+        };
+        for (long long cell = 0; cell < N; cell++)
+            for (int k = off[cell]; k < off[cell + 1]; k++) {
+                const int nb = adj[k] >> 1;
+                const bool selfL = (adj[k] & 1) == 0;
+                const double dL = selfL ? m->cellgeom[cell] : m->cellgeom[nb];
+                const double dR = selfL ? m->cellgeom[nb] : m->cellgeom[cell];
+                const double vL = vol(dL), vR = vol(dR);
+                fg[k] = 2.0 * (dL < dR ? vL / dL : vR / dR) / (dL + dR);
+            }
+        HIPCHK(hipMemcpy(c->fgeo, fg.data(), sizeof(double) * fg.size(), hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemcpy(c->bcell_off, boff.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->bface_id, bid.data(), sizeof(int32_t) * bid.size(), hipMemcpyHostToDevice));
     if (N) HIPCHK(hipMemcpy(c->cellgeom, m->cellgeom, sizeof(double) * N, hipMemcpyHostToDevice));
@@ -438,6 +464,7 @@ int cipher_rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active,
     L.cellgeom = c->cellgeom;
     L.cell_off = c->cell_off;
     L.adj = c->adj;
+    L.fgeo = c->fgeo;
     L.bcell_off = c->bcell_off;
     L.bface_id = c->bface_id;
     L.cm = c->cm;

@@ -352,6 +352,7 @@ extern "C" int cipher_batch_create(cipher_ctx *const *members, int32_t nmembers,
         a.cellgeom = c->cellgeom;
         a.cell_off = c->cell_off;
         a.adj = c->adj;
+        a.fgeo = c->fgeo;
         a.bcell_off = c->bcell_off;
         a.bface_id = c->bface_id;
         a.cm = (const double *)c->cm;

@@ -79,6 +79,7 @@ struct cipher_ctx {
     long long ncells = 0;
     int32_t *cell_off = nullptr, *adj = nullptr, *bcell_off = nullptr, *bface_id = nullptr;
     double *cellgeom = nullptr;
+    double *fgeo = nullptr;         // flux factor per face-list entry
     void *cm = nullptr;
     int32_t *glist = nullptr;
     int32_t *list = nullptr;        // FP32-mixed recomputation lists (5 * ncells)

@@ -13,6 +13,7 @@ struct RhsArgs {
     const double *cellgeom;
     const int32_t *cell_off;  // ncells+1: faces of each cell in face order
     const int32_t *adj;       // (neighbour << 1) | (self is R)
+    const double *fgeo;       // per adj entry: the face's flux factor 2*area/(hL + hR)
     const int32_t *bcell_off; // ncells+1: boundary faces of each cell
     const int32_t *bface_id;  // boundary label id per boundary-face entry
     const double *cm;         // per-cell [comp(ncp), mob(ncp), tcond]
@@ -45,10 +46,11 @@ constexpr int RHS_RING = 64;
 constexpr int BIN_CPT = 8;
 constexpr int BIN_TILE = 256 * BIN_CPT;
 
-// LDS bytes of a k_rhs block of cpb cells of G lanes
+// LDS bytes of a k_rhs block of cpb cells of G lanes: the interface type of
+// every slot pair, and energy, width and mobility of every pair gk < gj
 static inline size_t rhs_lds_bytes(const DevCfg &c, int G, int cpb) {
-    // pair interface types; with gradients also the pair energies/mobilities
-    return (size_t)cpb * G * G * (sizeof(int) + (c.has_grad ? 2 * sizeof(double) : 0));
+    const int np = G > 1 ? G * (G - 1) / 2 : 1;
+    return (size_t)cpb * ((size_t)G * G * sizeof(int) + 3 * (size_t)np * sizeof(double));
 }
 
 // Block ranges of one merged launch over the per-size lists: range r covers
@@ -71,6 +73,7 @@ struct RhsLaunch {
     uint32_t *active;
     const double *cellgeom;
     const int32_t *cell_off, *adj, *bcell_off, *bface_id;
+    const double *fgeo;
     void *cm;
     int32_t *bins;            // 5 lists of ncells entries (cells by active-set size)
     int32_t *counts;          // RHS_COUNTERS counters

@@ -409,6 +409,14 @@ __device__ __forceinline__ R pair_m(const DevCfg &c, const double *itab, int it,
     return m;
 }
 
+// pairs gk < gj of G slots, and the position of pair (gk, gj) in a
+// row-major upper triangle
+__host__ __device__ constexpr int pair_count(int G) { return G > 1 ? G * (G - 1) / 2 : 1; }
+template <int G>
+__device__ __forceinline__ int pair_index(int gk, int gj) {
+    return gk * (2 * G - gk - 1) / 2 + (gj - gk - 1);
+}
+
 // make LDS writes of this wave visible to its other lanes
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -552,7 +560,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
         }
         // a single-phase cell uses its faces only for chemistry and conduction
         // (its phase-fraction rate is zero), so grain-growth cells skip them
-        const bool need_lap = n > 1;
+        const bool need_lap = G > 1 && n > 1;   // n <= G: a one-lane list is single-phase
         const int kb = a.cell_off[cell];
         const int ke = (need_lap || NC > 1 || c.has_tcond) ? a.cell_off[cell + 1] : kb;
         // least-squares gradient of the slot's phase fraction
@@ -570,10 +578,12 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
         // chunk's work
         int e_n1 = (kb + slot < ke) ? a.adj[kb + slot] : 0;
         int e_n2 = (kb + G + slot < ke) ? a.adj[kb + G + slot] : 0;
-        double hN_n = hS, nbnd_n = 0.0, id0d_n = 0.0;
+        // the cell's volume (the L cell's when it is L, the R cell's when R)
+        const double vS = dfastpow(hS, (unsigned)dim);
+        double ff_n = 0.0, nbnd_n = 0.0, id0d_n = 0.0;
         if (kb + slot < ke) {
             const double *xq = a.X + (long long)(e_n1 >> 1) * D;
-            hN_n = a.cellgeom[e_n1 >> 1];
+            ff_n = a.fgeo[kb + slot];
             nbnd_n = xq[c.aso];
             id0d_n = xq[c.aso + 1];
         }
@@ -582,7 +592,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
             // -- face-parallel part
             const bool fon = slot < nfc;
             const int e = fon ? e_n1 : 0;
-            const double hN = fon ? hN_n : hS;
+            const double ffd = fon ? ff_n : 0.0;
             const double nbnd = fon ? nbnd_n : 0.0;
             const double id0d = fon ? id0d_n : 0.0;
             // next chunk: its scalars from the entries loaded one chunk ago,
@@ -590,7 +600,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
             e_n1 = e_n2;
             if (f0 + G + slot < ke) {
                 const double *xq = a.X + (long long)(e_n1 >> 1) * D;
-                hN_n = a.cellgeom[e_n1 >> 1];
+                ff_n = a.fgeo[f0 + G + slot];
                 nbnd_n = xq[c.aso];
                 id0d_n = xq[c.aso + 1];
             }
@@ -598,11 +608,8 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
             const int nbf = e >> 1;
             const bool selfL = (e & 1) == 0;
             const double *xf = a.X + (long long)nbf * D;
-            const double dL = selfL ? hS : hN, dR = selfL ? hN : hS;
-            const double vL = dfastpow(dL, (unsigned)dim), vR = dfastpow(dR, (unsigned)dim);
-            const double ffd = 2.0 * (dL < dR ? vL / dL : vR / dR) / (dL + dR);
             const R ff = (R)ffd;
-            const R vol = (R)(selfL ? vL : vR);
+            const R vol = (R)vS;
             const int nbn = clamp_count(nbnd, c.P);
             const int Ln = selfL ? n : nbn;
             const int Lid0 = selfL ? id0 : clamp_id(id0d, c.npf);
@@ -645,7 +652,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                         dmu[q] = selfL ? (mun - mus) : (mus - mun);
                     }
                     dmatvec<ND, double>(fd, MB, dmu);
-                    const double vold = selfL ? vL : vR;
+                    const double vold = vS;
 #pragma unroll
                     for (int q = 0; q < ND; q++) cterm[q] = selfL ? ffd * fd[q] / vold : -(ffd * fd[q] / vold);
                 }
@@ -820,12 +827,24 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
             for (int gj = 0; gj < G; gj++)
                 if (on && gj > slot && gj < n) pt[slot * G + gj] = row[gj];
         }
+        // energy, width and mobility of every active pair gk < gj, row gk
+        // written by lane gk (the pair loops below read them from LDS
+        // instead of chasing interface type -> table entry through memory)
+        R *ptab = reinterpret_cast<R *>(ptype + CPB * G * G) + lc * 3 * pair_count(G);
+        #pragma unroll 1
+        for (int gj = slot + 1; gj < n; gj++) {
+            const int it = pt[slot * G + gj];
+            R *pp = ptab + 3 * pair_index<G>(slot, gj);
+            pp[1] = (R)itab[3 * it + 1];
+            if (!grad_on) {
+                pp[0] = pair_e<R>(c, itab, it, Td);
+                pp[2] = pair_m<R>(c, itab, it, Td);
+            }
+        }
         // anisotropic interface energy and mobility of every active pair
         // (src/cipher.c:429-469): 1 + sum_dir (n.dir)^4 val along the unit
         // difference of the two gradients, then the constant and the
         // activation factor; row gk of the group's tables
-        double *pe_tab = (double *)(ptype + CPB * G * G) + lc * G * G * 2;
-        double *pm_tab = pe_tab + G * G;
         if (grad_on) {
             #pragma unroll 1
             for (int gj = 0; gj < n; gj++) {
@@ -856,8 +875,8 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                     if (c.has_iact && c.ts[f.e_act].n) ev *= dexp<double>(dts_w(c, f.e_act, Td, iso) / CRG / Td);
                     mv *= f.m0;
                     if (c.has_iact && c.ts[f.m_act].n) mv *= dexp<double>(dts_w(c, f.m_act, Td, iso) / CRG / Td);
-                    pe_tab[slot * G + gj] = ev;
-                    pm_tab[slot * G + gj] = mv;
+                    ptab[3 * pair_index<G>(slot, gj)] = (R)ev;
+                    ptab[3 * pair_index<G>(slot, gj) + 2] = (R)mv;
                 }
             }
         }
@@ -866,7 +885,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
         double pfrhs = 0.0;
         bool act = false;
         bool unsure = false;
-        if (n > 1) {
+        if (G > 1 && n > 1) {
             // capillary driving force (src/cipher.c:474-497), replayed per slot
             R cap = R(0), capabs = R(0);
             #pragma unroll 1
@@ -877,9 +896,9 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                 for (int gj = gk + 1; gj < n; gj++) {
                     const R phj = gsh<G>(phi, gj), lapj = gsh<G>(plap, gj);
                     const int idj = gsh<G>(myid, gj);
-                    const int it = pt[gk * G + gj];
-                    const R e = grad_on ? (R)pe_tab[gk * G + gj] : pair_e<R>(c, itab, it, Td);
-                    const R w = (R)itab[3 * it + 1];
+                    const R *pkj = ptab + 3 * pair_index<G>(gk, gj);
+                    const R e = pkj[0];
+                    const R w = pkj[1];
                     R lapaj = R(0), lapak = R(0);
                     if constexpr (MIXED) { lapaj = gsh<G>(lapabs, gj); lapak = gsh<G>(lapabs, gk); }
                     if (slot == gk) {
@@ -894,14 +913,14 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                     for (int gi = gj + 1; gi < n; gi++) {
                         const R phi_i = gsh<G>(phi, gi);
                         const int idi = gsh<G>(myid, gi);
-                        const int it2 = pt[gk * G + gi];
-                        const int it3 = pt[gj * G + gi];
+                        const R *pki = ptab + 3 * pair_index<G>(gk, gi);
+                        const R *pji = ptab + 3 * pair_index<G>(gj, gi);
                         R tj = R(0);
                         const R r1 = e / w;
                         tj = tj > r1 ? tj : r1;
-                        const R r2 = (grad_on ? (R)pe_tab[gk * G + gi] : pair_e<R>(c, itab, it2, Td)) / (R)itab[3 * it2 + 1];
+                        const R r2 = pki[0] / pki[1];
                         tj = tj > r2 ? tj : r2;
-                        const R r3 = (grad_on ? (R)pe_tab[gj * G + gi] : pair_e<R>(c, itab, it3, Td)) / (R)itab[3 * it3 + 1];
+                        const R r3 = pji[0] / pji[1];
                         tj = tj > r3 ? tj : r3;
                         tj *= (R)c.junctionpenalty;
                         R t3 = R(0);
@@ -997,7 +1016,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                     const R capj = gsh<G>(cap, gj);
                     const double chj = gsh<G>(chs, gj), hj = gsh<G>(hd, gj);
                     const int idj = gsh<G>(myid, gj);
-                    const R mkj = grad_on ? (R)pm_tab[gk * G + gj] : pair_m<R>(c, itab, pt[gk * G + gj], Td);
+                    const R mkj = ptab[3 * pair_index<G>(gk, gj) + 2];
                     const double rv = (double)mkj * ((double)(capk - capj) - (chk - chj) * 8.0 * sqrt(hk * hj) / CPI);
                     if (slot == gk) ru += rv;
                     if (slot == gj) ru -= rv;
@@ -1034,7 +1053,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                     const double chj = gsh<G>(chs, gj), hj = gsh<G>(hd, gj);
                     const int idj = gsh<G>(myid, gj);
                     if (!aj) continue;
-                    const R mkj = grad_on ? (R)pm_tab[gk * G + gj] : pair_m<R>(c, itab, pt[gk * G + gj], Td);
+                    const R mkj = ptab[3 * pair_index<G>(gk, gj) + 2];
                     const double rv = (double)mkj * ((double)(capk - capj) - (chk - chj) * 8.0 * sqrt(hk * hj) / CPI);
                     if (slot == gk) pfrhs += rv;
                     if (slot == gj) pfrhs -= rv;
@@ -1156,7 +1175,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
             double wdp[ND];
 #pragma unroll
             for (int k = 0; k < ND; k++) wdp[k] = 0.0;
-            if (n > 1) {
+            if (G > 1 && n > 1) {
 #pragma unroll
                 for (int q = 0; q < ND; q++) {
                     double wpf = 0.0;
@@ -1431,7 +1450,7 @@ cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st) {
     const bool need_prep = (NC > 1) || c.has_tcond;
     RhsArgs a;
     a.X = L.X; a.F = L.F; a.active = L.active; a.cellgeom = L.cellgeom;
-    a.cell_off = L.cell_off; a.adj = L.adj; a.bcell_off = L.bcell_off; a.bface_id = L.bface_id;
+    a.cell_off = L.cell_off; a.adj = L.adj; a.fgeo = L.fgeo; a.bcell_off = L.bcell_off; a.bface_id = L.bface_id;
     a.cm = (const double *)L.cm; a.ncells = N;
     a.cells = nullptr; a.cells_count = nullptr;
     a.redo = nullptr; a.redo_count = nullptr; a.map = 0; a.pad_ = 0;
