@@ -226,6 +226,12 @@ static int build_tables(cipher_batch *b) {
                  return list_blocks(b->m[i]->ncells, Nclass, 256 / L.shape.lanes[q], k.shape[q].cap);
              })))
             return rc;
+        L.relists.lds = L.lists.lds;
+        if ((rc = make_list_segs(b, L.relists, k.members, order, [&](int i, int q) {
+                 return list_blocks(b->m[i]->ncells, Nclass, 256 / L.shape.lanes[q],
+                                    std::min(k.shape[q].cap, RHS_REDO_CAP));
+             })))
+            return rc;
     }
     return CIPHER_OK;
 }

@@ -42,6 +42,9 @@ constexpr int RHS_COUNTERS = 16;
 // counter sets that eager right-hand sides take in turn between clears
 constexpr int RHS_RING = 64;
 
+// blocks per list of the FP32-mixed recomputation launch
+constexpr int RHS_REDO_CAP = 512;
+
 // cells per binning tile: 256 threads of BIN_CPT consecutive cells
 constexpr int BIN_CPT = 8;
 constexpr int BIN_TILE = 256 * BIN_CPT;
@@ -170,6 +173,7 @@ struct BatchLaunch {
     int ncp, G, nbins;
     int do_bin;               // sort the cells into lists (0: reuse the previous call's lists)
     BatchSegs fold, prep, bin, lists, post, dofs;
+    BatchSegs relists;        // the list launch's segments with at most RHS_REDO_CAP blocks per list
     BatchLists shape;
     const double *X;
     double *F;

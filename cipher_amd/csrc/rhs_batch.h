@@ -104,8 +104,9 @@ cipher_status batch_G(const BatchLaunch &L, int precision, hipStream_t st) {
         if (precision == CIPHER_FP32_MIXED) {
             hipLaunchKernelGGL((k_rhs_batch<G, NC, float>), dim3((unsigned)s.nblocks), dim3(256), s.lds, st, L.mem,
                                L.live, s.seg, s.segmem, s.segq, s.nseg, L.shape, 0, L.X, L.F, L.active);
-            hipLaunchKernelGGL((k_rhs_batch<G, NC, double>), dim3((unsigned)s.nblocks), dim3(256), s.lds, st, L.mem,
-                               L.live, s.seg, s.segmem, s.segq, s.nseg, L.shape, 1, L.X, L.F, L.active);
+            const BatchSegs &r = L.relists;
+            hipLaunchKernelGGL((k_rhs_batch<G, NC, double>), dim3((unsigned)r.nblocks), dim3(256), r.lds, st, L.mem,
+                               L.live, r.seg, r.segmem, r.segq, r.nseg, L.shape, 1, L.X, L.F, L.active);
         } else {
             hipLaunchKernelGGL((k_rhs_batch<G, NC, double>), dim3((unsigned)s.nblocks), dim3(256), s.lds, st, L.mem,
                                L.live, s.seg, s.segmem, s.segq, s.nseg, L.shape, 0, L.X, L.F, L.active);

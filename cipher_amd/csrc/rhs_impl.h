@@ -580,6 +580,8 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
         int e_n2 = (kb + G + slot < ke) ? a.adj[kb + G + slot] : 0;
         // the cell's volume (the L cell's when it is L, the R cell's when R)
         const double vS = dfastpow(hS, (unsigned)dim);
+        const R vol = (R)vS;
+        const R ivol = MIXED ? (R)(1.0 / vS) : R(0);    // FP32-mixed: one division per cell
         double ff_n = 0.0, nbnd_n = 0.0, id0d_n = 0.0;
         if (kb + slot < ke) {
             const double *xq = a.X + (long long)(e_n1 >> 1) * D;
@@ -609,7 +611,6 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
             const bool selfL = (e & 1) == 0;
             const double *xf = a.X + (long long)nbf * D;
             const R ff = (R)ffd;
-            const R vol = (R)vS;
             const int nbn = clamp_count(nbnd, c.P);
             const int Ln = selfL ? n : nbn;
             const int Lid0 = selfL ? id0 : clamp_id(id0d, c.npf);
@@ -732,17 +733,23 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                     pv = (R)pvd;
                     if (grad_on && on && pos >= 0)
                         for (int d = 0; d < c.dim; d++) gr[d] += c.gw[(long long)(f0 + t) * c.dim + d] * pvd;
-                    const R fft = gsh<G>(ff, t), volt = gsh<G>(vol, t);
+                    const R fft = gsh<G>(ff, t);
                     const int sL = gsh<G>((int)selfL, t);
                     const int lgt = gsh<G>(lguard, t);     // every lane of the group shuffles
                     if (on && pos >= 0 && lgt) {
                         R term;
-                        if (sL) {
+                        if constexpr (MIXED) {
+                            // the difference is taken in FP64 and rounded once,
+                            // so each term carries relative errors only (at most
+                            // 5 roundings: difference, two factors, two products)
+                            const R fl = sL ? (R)(pvd - phid) : (R)(phid - pvd);
+                            term = sL ? fft * fl * ivol : -(fft * fl * ivol);
+                        } else if (sL) {
                             const R fl = pv - phi;
-                            term = fft * fl / volt;
+                            term = fft * fl / vol;
                         } else {
                             const R fl = phi - pv;
-                            term = -(fft * fl / volt);
+                            term = -(fft * fl / vol);
                         }
                         lap.add(term);
                         if constexpr (MIXED) lapabs += fabsf(term);
@@ -990,14 +997,6 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                     }
                 }
             }
-            R chsabs = R(0);
-            if constexpr (MIXED) {
-                // magnitude of the inputs of the interpolant-derivative product
-                R mx = R(0);
-                #pragma unroll 1
-                for (int g = 0; g < n; g++) mx = fmaxf(mx, (float)fabs(gsh<G>(chs, g)));
-                chsabs = R(12) * mx;
-            }
             chs = interp_deriv<G, double>(c, n, phid, hhd, chs);
             // unconstrained rate and active flags (src/cipher.c:524-539); the
             // pair rates are summed in FP64 in both modes
@@ -1017,23 +1016,34 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                     const double chj = gsh<G>(chs, gj), hj = gsh<G>(hd, gj);
                     const int idj = gsh<G>(myid, gj);
                     const R mkj = ptab[3 * pair_index<G>(gk, gj) + 2];
-                    const double rv = (double)mkj * ((double)(capk - capj) - (chk - chj) * 8.0 * sqrt(hk * hj) / CPI);
+                    const double rin = (double)(capk - capj) - (chk - chj) * 8.0 * sqrt(hk * hj) / CPI;
+                    const double rv = (double)mkj * rin;
                     if (slot == gk) ru += rv;
                     if (slot == gj) ru -= rv;
                     if constexpr (MIXED) {
+                        // the chemical term is FP64 in both modes (identical
+                        // bits); the FP32 errors are those of the two capillary
+                        // terms (bounded by their magnitudes), of their
+                        // difference and of the mobility's rounding
                         const R capaj = gsh<G>(capabs, gj);
-                        if (slot == gk || slot == gj)
-                            ruabs += fabsf(mkj) * (fabsf(capk) + fabsf(capj) + capak + capaj
-                                                   + ((float)fabs(chk) + (float)fabs(chj) + R(2) * chsabs) * R(8)
-                                                         * (float)sqrt(hk * hj) / (R)CPI);
+                        if (slot == gk || slot == gj) ruabs += fabsf(mkj) * (capak + capaj + (float)fabs(rin));
                     }
                 }
                 if (slot == gk) {
                     act = (phid > CIPHER_TOL && phid < 1.0 - CIPHER_TOL) || (phid < CIPHER_TOL && ru > 0.0)
                           || (phid > 1.0 - CIPHER_TOL && ru < 0.0);
-                    // the decision reads the sign of ru: is it inside the FP32 band?
-                    if constexpr (MIXED)
-                        unsure = !(phid > CIPHER_TOL && phid < 1.0 - CIPHER_TOL) && (float)fabs(ru) <= 1.0e-4f * ruabs;
+                    // the decision reads the sign of ru: is it inside the FP32
+                    // band?  |ru32 - ru64| <= (25 + m) u ruabs, u = 2^-24, m the
+                    // accumulation steps of a capillary term (pairs and
+                    // junctions of the cell): 7.1u from the compensated
+                    // Laplacian, 16.3u from a pair term, m u from summing them,
+                    // u from the difference and the mobility; the band takes
+                    // (32 + 2m) u
+                    if constexpr (MIXED) {
+                        const int msum = (n - 1) + (n - 1) * (n - 2) / 2;
+                        const float band = (float)(32 + 2 * msum) * 5.9604644775390625e-8f;
+                        unsure = !(phid > CIPHER_TOL && phid < 1.0 - CIPHER_TOL) && (float)fabs(ru) <= band * ruabs;
+                    }
                 }
             }
             #pragma unroll 1
@@ -1490,10 +1500,20 @@ cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st) {
     if (precision == CIPHER_FP32_MIXED) {
         hipLaunchKernelGGL((k_rhs_all<G, NC, float>), dim3((unsigned)tot), dim3(256), lds, st, a, c, L.bins, L.redo,
                            L.counts, rg);
-        // FP64 recomputation of the cells whose flags sat in the FP32 band
-        rg.redo = 1;
-        hipLaunchKernelGGL((k_rhs_all<G, NC, double>), dim3((unsigned)tot), dim3(256), lds, st, a, c, L.bins, L.redo,
-                           L.counts, rg);
+        // FP64 recomputation of the cells whose flags sat in the FP32 band;
+        // the recomputation lists are short, so each range gets at most
+        // RHS_REDO_CAP blocks (they stride over longer lists)
+        ListShape rs[5];
+        for (int q = 0; q < 5; q++) {
+            rs[q] = L.shape[q];
+            rs[q].cap = std::min(rs[q].cap, RHS_REDO_CAP);
+        }
+        size_t lds2 = 0;
+        long long tot2 = 0;
+        ListRanges rr = make_ranges(rs, N, c, nbins, &lds2, &tot2);
+        rr.redo = 1;
+        hipLaunchKernelGGL((k_rhs_all<G, NC, double>), dim3((unsigned)tot2), dim3(256), lds2, st, a, c, L.bins, L.redo,
+                           L.counts, rr);
     } else {
         hipLaunchKernelGGL((k_rhs_all<G, NC, double>), dim3((unsigned)tot), dim3(256), lds, st, a, c, L.bins, L.redo,
                            L.counts, rg);
