@@ -801,19 +801,14 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
         const R plap = lap.s;
 
         // ---- cell (src/cipher.c:392-664) ------------------------------------
-        R hh, h;
-        interpolant<G, R>(c, n, phi, hh, h);
-        if (!on) { h = R(0); hh = R(0); }
-        // FP64 interpolants for the chemical driving force: chemical energies
-        // (~1e9 J/m^3) nearly cancel between phases, beyond FP32's reach
+        // interpolants in FP64 in both modes: the chemical driving force and
+        // the composition rate need them (chemical energies of ~1e9 J/m^3
+        // nearly cancel between phases, beyond FP32's reach); FP32-mixed
+        // rounds them for its FP32 terms
         double hhd, hd;
-        if constexpr (MIXED) {
-            interpolant<G, double>(c, n, phid, hhd, hd);
-            if (!on) { hd = 0.0; hhd = 0.0; }
-        } else {
-            hhd = hh;
-            hd = h;
-        }
+        interpolant<G, double>(c, n, phid, hhd, hd);
+        if (!on) { hd = 0.0; hhd = 0.0; }
+        const R h = (R)hd;
         const int mat = on ? c.phase_material[myid] : 0;
         const cipher_material &m = c.mat[mat];
         const int ns = on ? m.nsites : 0;
@@ -837,15 +832,23 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
         // energy, width and mobility of every active pair gk < gj, row gk
         // written by lane gk (the pair loops below read them from LDS
         // instead of chasing interface type -> table entry through memory)
-        R *ptab = reinterpret_cast<R *>(ptype + CPB * G * G) + lc * 3 * pair_count(G);
+        // (FP32-mixed also keeps 1/width and width/pi^2, rounded once from
+        // FP64, in place of the FP32 divisions of the pair loops)
+        constexpr int PT = MIXED ? 5 : 3;                  // entries per pair
+        R *ptab = reinterpret_cast<R *>(ptype + CPB * G * G) + lc * PT * pair_count(G);
         #pragma unroll 1
         for (int gj = slot + 1; gj < n; gj++) {
             const int it = pt[slot * G + gj];
-            R *pp = ptab + 3 * pair_index<G>(slot, gj);
-            pp[1] = (R)itab[3 * it + 1];
+            R *pp = ptab + PT * pair_index<G>(slot, gj);
+            const double wd = itab[3 * it + 1];
+            pp[1] = (R)wd;
             if (!grad_on) {
                 pp[0] = pair_e<R>(c, itab, it, Td);
                 pp[2] = pair_m<R>(c, itab, it, Td);
+            }
+            if constexpr (MIXED) {
+                pp[3] = (R)(1.0 / wd);
+                pp[4] = (R)(wd / (CPI * CPI));
             }
         }
         // anisotropic interface energy and mobility of every active pair
@@ -882,8 +885,8 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                     if (c.has_iact && c.ts[f.e_act].n) ev *= dexp<double>(dts_w(c, f.e_act, Td, iso) / CRG / Td);
                     mv *= f.m0;
                     if (c.has_iact && c.ts[f.m_act].n) mv *= dexp<double>(dts_w(c, f.m_act, Td, iso) / CRG / Td);
-                    ptab[3 * pair_index<G>(slot, gj)] = (R)ev;
-                    ptab[3 * pair_index<G>(slot, gj) + 2] = (R)mv;
+                    ptab[PT * pair_index<G>(slot, gj)] = (R)ev;
+                    ptab[PT * pair_index<G>(slot, gj) + 2] = (R)mv;
                 }
             }
         }
@@ -903,31 +906,36 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                 for (int gj = gk + 1; gj < n; gj++) {
                     const R phj = gsh<G>(phi, gj), lapj = gsh<G>(plap, gj);
                     const int idj = gsh<G>(myid, gj);
-                    const R *pkj = ptab + 3 * pair_index<G>(gk, gj);
+                    const R *pkj = ptab + PT * pair_index<G>(gk, gj);
                     const R e = pkj[0];
                     const R w = pkj[1];
-                    R lapaj = R(0), lapak = R(0);
-                    if constexpr (MIXED) { lapaj = gsh<G>(lapabs, gj); lapak = gsh<G>(lapabs, gk); }
-                    if (slot == gk) {
-                        cap -= e * (phj / w + lapj * w / (R)CPI / (R)CPI);
-                        if constexpr (MIXED) capabs += fabsf(e) * (fabsf(phj / w) + lapaj * fabsf(w) / (R)CPI / (R)CPI);
-                    }
-                    if (slot == gj) {
-                        cap -= e * (phk / w + lapk * w / (R)CPI / (R)CPI);
-                        if constexpr (MIXED) capabs += fabsf(e) * (fabsf(phk / w) + lapak * fabsf(w) / (R)CPI / (R)CPI);
+                    if constexpr (MIXED) {
+                        const R iw = pkj[3], wp = pkj[4];
+                        const R lapaj = gsh<G>(lapabs, gj), lapak = gsh<G>(lapabs, gk);
+                        if (slot == gk) {
+                            cap -= e * (phj * iw + lapj * wp);
+                            capabs += fabsf(e) * (fabsf(phj * iw) + lapaj * fabsf(wp));
+                        }
+                        if (slot == gj) {
+                            cap -= e * (phk * iw + lapk * wp);
+                            capabs += fabsf(e) * (fabsf(phk * iw) + lapak * fabsf(wp));
+                        }
+                    } else {
+                        if (slot == gk) cap -= e * (phj / w + lapj * w / (R)CPI / (R)CPI);
+                        if (slot == gj) cap -= e * (phk / w + lapk * w / (R)CPI / (R)CPI);
                     }
                     #pragma unroll 1
                     for (int gi = gj + 1; gi < n; gi++) {
                         const R phi_i = gsh<G>(phi, gi);
                         const int idi = gsh<G>(myid, gi);
-                        const R *pki = ptab + 3 * pair_index<G>(gk, gi);
-                        const R *pji = ptab + 3 * pair_index<G>(gj, gi);
+                        const R *pki = ptab + PT * pair_index<G>(gk, gi);
+                        const R *pji = ptab + PT * pair_index<G>(gj, gi);
                         R tj = R(0);
-                        const R r1 = e / w;
+                        const R r1 = MIXED ? e * pkj[3] : e / w;
                         tj = tj > r1 ? tj : r1;
-                        const R r2 = pki[0] / pki[1];
+                        const R r2 = MIXED ? pki[0] * pki[3] : pki[0] / pki[1];
                         tj = tj > r2 ? tj : r2;
-                        const R r3 = pji[0] / pji[1];
+                        const R r3 = MIXED ? pji[0] * pji[3] : pji[0] / pji[1];
                         tj = tj > r3 ? tj : r3;
                         tj *= (R)c.junctionpenalty;
                         R t3 = R(0);
@@ -1015,7 +1023,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                     const R capj = gsh<G>(cap, gj);
                     const double chj = gsh<G>(chs, gj), hj = gsh<G>(hd, gj);
                     const int idj = gsh<G>(myid, gj);
-                    const R mkj = ptab[3 * pair_index<G>(gk, gj) + 2];
+                    const R mkj = ptab[PT * pair_index<G>(gk, gj) + 2];
                     const double rin = (double)(capk - capj) - (chk - chj) * 8.0 * sqrt(hk * hj) / CPI;
                     const double rv = (double)mkj * rin;
                     if (slot == gk) ru += rv;
@@ -1063,7 +1071,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                     const double chj = gsh<G>(chs, gj), hj = gsh<G>(hd, gj);
                     const int idj = gsh<G>(myid, gj);
                     if (!aj) continue;
-                    const R mkj = ptab[3 * pair_index<G>(gk, gj) + 2];
+                    const R mkj = ptab[PT * pair_index<G>(gk, gj) + 2];
                     const double rv = (double)mkj * ((double)(capk - capj) - (chk - chj) * 8.0 * sqrt(hk * hj) / CPI);
                     if (slot == gk) pfrhs += rv;
                     if (slot == gj) pfrhs -= rv;
@@ -1126,7 +1134,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                 }
             }
             // composition rate (src/cipher.c:584-662), FP64 in both modes
-            const double hq = (double)h, hhq = (double)hh, pfq = (double)pfrhs, tmq = (double)tmrhs;
+            const double hq = hd, hhq = hhd, pfq = (double)pfrhs, tmq = (double)tmrhs;
             double wcp[NDV];
 #pragma unroll
             for (int k = 0; k < ND; k++) wcp[k] = 0.0;
