@@ -50,7 +50,14 @@ struct BatchClass {
     std::vector<int> members;
     ListShape shape[5];
     BatchLaunch L{};
+    double cost = 0.0;        // estimated time of the class's launches alone (ms)
+    int lane = 0;             // stream slot of the class (0: the caller's stream)
 };
+
+// streams a batched right-hand side spreads its classes over: the caller's
+// and BATCH_STREAMS - 1 forked ones (a process has 4 hardware queues by
+// default; a fifth stream would share one and serialise behind it)
+constexpr int BATCH_STREAMS = 4;
 
 struct BatchTs {
     bool ready = false;
@@ -90,6 +97,7 @@ struct cipher_batch {
     double *h_norms = nullptr, *h_dt = nullptr;  // pinned, nmembers
     int32_t *h_err = nullptr, *h_live = nullptr, *h_acc = nullptr;
     std::vector<BatchClass> cls;
+    std::vector<int> sched;                 // class enqueue order (longest first)
     BatchLaunch all{};                      // fold and per-dof launches over every member
     std::vector<void *> segs;               // device segment tables
     hipStream_t aux[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -98,6 +106,35 @@ struct cipher_batch {
 };
 
 static int lanes_for(int P) { return P <= 4 ? 4 : (P <= 8 ? 8 : (P <= 16 ? 16 : 0)); }
+
+// Longest-processing-time schedule of the classes over the streams: classes
+// in decreasing cost, each to the stream with the least cost so far, so the
+// streams finish together instead of one stream's queue of small classes
+// running alone at the end.
+static void schedule_classes(cipher_batch *b) {
+    const int nk = (int)b->cls.size();
+    b->sched.resize(nk);
+    for (int k = 0; k < nk; k++) b->sched[k] = k;
+    std::stable_sort(b->sched.begin(), b->sched.end(),
+                     [&](int x, int y) { return b->cls[x].cost > b->cls[y].cost; });
+    double load[BATCH_STREAMS] = {0.0};
+    for (int k : b->sched) {
+        int s = 0;
+        for (int j = 1; j < BATCH_STREAMS; j++)
+            if (load[j] < load[s]) s = j;
+        b->cls[k].lane = s;
+        load[s] += b->cls[k].cost;
+    }
+}
+
+// cost model used until the classes are timed: cells times a per-cell cost
+// that grows with the component count and the lane capacity
+static double model_cost(const cipher_batch *b, const BatchClass &k) {
+    long long cells = 0;
+    for (int i : k.members) cells += b->m[i]->ncells;
+    const double w = (k.ncp == 1 ? 0.5 : 0.8 * k.ncp) * (k.G >= 16 ? 1.5 : 1.0);
+    return 1e-6 * w * (double)cells;
+}
 static bool needs_prep(const cipher_ctx *c) { return c->L.dp_size > 0 || c->dc.has_tcond; }
 
 static void free_segs(cipher_batch *b) {
@@ -399,6 +436,8 @@ extern "C" int cipher_batch_create(cipher_ctx *const *members, int32_t nmembers,
         std::string msg = cipher_last_error();
         return bail(rc, msg);
     }
+    for (BatchClass &k : b->cls) k.cost = model_cost(b, k);
+    schedule_classes(b);
     *out = b;
     return CIPHER_OK;
 }
@@ -481,23 +520,25 @@ static int batch_rhs_impl(cipher_batch *b, const double *X, double *F, uint32_t 
     if (launch_fold_batch(b->all, st)) return cipher_fail(CIPHER_ERR_HIP, "batch fold launch");
     // classes are independent: they run on the caller's stream and four
     // forked streams, each class's launches in order on its stream
+    // (in the schedule's order: longest first, each on its stream slot)
     const bool fork = b->cls.size() > 1;
+    const int nfork = std::min<int>(BATCH_STREAMS - 1, (int)b->cls.size() - 1);
     if (fork) {
         BCHK(hipEventRecord(b->ev_fork, st));
-        for (int i = 0; i < 4; i++) BCHK(hipStreamWaitEvent(b->aux[i], b->ev_fork, 0));
+        for (int i = 0; i < nfork; i++) BCHK(hipStreamWaitEvent(b->aux[i], b->ev_fork, 0));
     }
-    for (size_t ki = 0; ki < b->cls.size(); ki++) {
+    for (int ki : b->sched) {
         BatchClass &k = b->cls[ki];
         k.L.X = X;
         k.L.F = F;
         k.L.active = active;
         k.L.live = live;
         k.L.do_bin = reuse ? 0 : 1;
-        hipStream_t s = (ki % 5 == 0) ? st : b->aux[ki % 5 - 1];
+        hipStream_t s = (!fork || k.lane == 0) ? st : b->aux[std::min(k.lane, nfork) - 1];
         if (launch_rhs_batch(k.L, b->precision, s)) return cipher_fail(CIPHER_ERR_HIP, "batch list launch");
     }
     if (fork) {
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < nfork; i++) {
             BCHK(hipEventRecord(b->ev_join[i], b->aux[i]));
             BCHK(hipStreamWaitEvent(st, b->ev_join[i], 0));
         }
@@ -558,6 +599,40 @@ extern "C" int cipher_batch_wrms_norm(cipher_batch *b, const double *U, const do
 }
 
 // ---- autotuner -------------------------------------------------------------
+// Time each class's launches alone on the caller's stream (median of reps,
+// counters cleared and T-series folded before each, untimed) and schedule the
+// classes over the streams by those times.
+static int measure_class_costs(cipher_batch *b, const double *X, double *F, int reps, hipEvent_t e0, hipEvent_t e1) {
+    hipStream_t st = b->stream;
+    const int nmem = (int)b->m.size();
+    int rc;
+    for (BatchClass &k : b->cls) {
+        std::vector<float> v;
+        for (int r = 0; r < reps; r++) {
+            if ((rc = fill(b, b->d_counts, (long long)RHS_COUNTERS * (long long)nmem, 0, st))) return rc;
+            b->all.X = X;
+            b->all.live = nullptr;
+            if (launch_fold_batch(b->all, st)) return cipher_fail(CIPHER_ERR_HIP, "batch fold launch");
+            k.L.X = X;
+            k.L.F = F;
+            k.L.active = nullptr;
+            k.L.live = nullptr;
+            k.L.do_bin = 1;
+            BCHK(hipEventRecord(e0, st));
+            if (launch_rhs_batch(k.L, b->precision, st)) return cipher_fail(CIPHER_ERR_HIP, "batch list launch");
+            BCHK(hipEventRecord(e1, st));
+            BCHK(hipEventSynchronize(e1));
+            float t = 0.f;
+            BCHK(hipEventElapsedTime(&t, e0, e1));
+            v.push_back(t);
+        }
+        std::sort(v.begin(), v.end());
+        k.cost = v[v.size() / 2];
+    }
+    schedule_classes(b);
+    return CIPHER_OK;
+}
+
 // Coordinate descent over the launch shape of each class's lists: for every
 // (class, list) in turn, lanes per cell, block order and grid cap are varied
 // one at a time, each candidate timed as the median of `reps`
@@ -588,7 +663,8 @@ extern "C" int cipher_batch_autotune(cipher_batch *b, const double *X, double *F
         return CIPHER_OK;
     };
     float best = 0.f;
-    if ((rc = batch_rhs_impl(b, X, F, nullptr, nullptr)) == CIPHER_OK) rc = time_once(best);
+    if ((rc = batch_rhs_impl(b, X, F, nullptr, nullptr)) == CIPHER_OK) rc = measure_class_costs(b, X, F, reps, e0, e1);
+    if (rc == CIPHER_OK) rc = time_once(best);
     for (size_t ki = 0; ki < b->cls.size() && rc == CIPHER_OK; ki++) {
         BatchClass &k = b->cls[ki];
         for (int q = 0; q < k.L.nbins && rc == CIPHER_OK; q++) {
@@ -608,6 +684,26 @@ extern "C" int cipher_batch_autotune(cipher_batch *b, const double *X, double *F
                 else k.shape[q] = keep;
             }
             if (rc == CIPHER_OK) rc = build_tables(b);
+        }
+    }
+    // the stream schedule of the tuned shapes (kept only if it is faster)
+    if (rc == CIPHER_OK) {
+        std::vector<double> keep_cost;
+        for (const BatchClass &k : b->cls) keep_cost.push_back(k.cost);
+        const std::vector<int> keep_sched = b->sched;
+        std::vector<int> keep_lane;
+        for (const BatchClass &k : b->cls) keep_lane.push_back(k.lane);
+        float ms = 0.f;
+        if ((rc = measure_class_costs(b, X, F, reps, e0, e1)) == CIPHER_OK && (rc = time_once(ms)) == CIPHER_OK) {
+            if (ms < best) {
+                best = ms;
+            } else {
+                b->sched = keep_sched;
+                for (size_t k = 0; k < b->cls.size(); k++) {
+                    b->cls[k].cost = keep_cost[k];
+                    b->cls[k].lane = keep_lane[k];
+                }
+            }
         }
     }
     (void)hipEventDestroy(e0);
