@@ -40,7 +40,7 @@
 #define CIPHER_RHS_MINB1 3
 #endif
 #ifndef CIPHER_RHS_MINBN
-#define CIPHER_RHS_MINBN 1
+#define CIPHER_RHS_MINBN 2
 #endif
 
 namespace cipher {
@@ -1133,6 +1133,20 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                         cprd[s * ND + k] = m.chempot_ex_kineticcoeff * (spw[s * ND + k] - x[c.exo + slot * c.SP + s * ND + k]);
                 }
             }
+            {
+                // the EX entries of the cell's F record (unused slots and sites 0)
+                double *Fx = a.F + cell * D + c.exo;
+                const int msites = c.SP / ND;
+                for (int p = slot; p < c.P; p += G) {
+                    const bool own = (p == slot) && on;
+#pragma unroll
+                    for (int s = 0; s < 2; s++) {
+                        if (s >= msites) break;
+#pragma unroll
+                        for (int q = 0; q < ND; q++) Fx[p * c.SP + s * ND + q] = (own && s < ns) ? cprd[s * ND + q] : 0.0;
+                    }
+                }
+            }
             // composition rate (src/cipher.c:584-662), FP64 in both modes
             const double hq = hd, hhq = hhd, pfq = (double)pfrhs, tmq = (double)tmrhs;
             double wcp[NDV];
@@ -1168,6 +1182,20 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                 if (iso) d_sitefrac_tangent<NC, double, true>(c, m, tpot, tT, exs, yd, spimd, Td);
                 else d_sitefrac_tangent<NC, double, false>(c, m, tpot, tT, exs, yd, spimd, Td);
             }
+            // the lane's explicit-potential tangent terms T_ex . EXdot per
+            // sublattice (evaluated once; the tangent arrays die after the
+            // composition tangent below)
+            double tv[2 * ND];
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                double tex[ND * ND];
+#pragma unroll
+                for (int q = 0; q < ND * ND; q++) tex[q] = exs == 0.0 ? 0.0 : -tpot[s * ND * ND + q];
+                double cps[ND];
+#pragma unroll
+                for (int q = 0; q < ND; q++) cps[q] = cprd[s * ND + q];
+                dmatvec<ND, double>(tv + s * ND, tex, cps);
+            }
             double dcdm[ND * ND];
 #pragma unroll
             for (int q = 0; q < ND * ND; q++) dcdm[q] = 0.0;
@@ -1190,6 +1218,8 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                 for (int k = 0; k < ND; k++) tmp[k] = wcp[k];
                 dmatvec<ND, double>(wcp, dcdm, tmp);
             }
+            double dmdc[ND * ND];
+            dinvert<ND, double>(dmdc, dcdm);
             double wdp[ND];
 #pragma unroll
             for (int k = 0; k < ND; k++) wdp[k] = 0.0;
@@ -1247,24 +1277,14 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                 const int nsg = gsh<G>(ns, g);
 #pragma unroll
                 for (int s = 0; s < 2; s++) {
-                    double tv[ND];
-                    double tex[ND * ND];
-#pragma unroll
-                    for (int q = 0; q < ND * ND; q++) tex[q] = exs == 0.0 ? 0.0 : -tpot[s * ND * ND + q];
-                    double cps[ND];
-#pragma unroll
-                    for (int q = 0; q < ND; q++) cps[q] = cprd[s * ND + q];
-                    dmatvec<ND, double>(tv, tex, cps);
                     const double st = (s < ns) ? m.stoichiometry[s] : 0.0;
 #pragma unroll
                     for (int q = 0; q < ND; q++) {
-                        const double v = gsh<G>(hq * st * (tv[q] + tT[s * ND + q] * tmq), g);
+                        const double v = gsh<G>(hq * st * (tv[s * ND + q] + tT[s * ND + q] * tmq), g);
                         if (s < nsg) wdp[q] -= v;
                     }
                 }
             }
-            double dmdc[ND * ND];
-            dinvert<ND, double>(dmdc, dcdm);
             dmatvec<ND, double>(dprhs, dmdc, wdp);
         }
 
@@ -1276,19 +1296,6 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
             Fc[c.tmo] = (double)tmrhs;
 #pragma unroll
             for (int q = 0; q < ND; q++) Fc[c.dpo + q] = dprhs[q];
-        }
-        if constexpr (NC > 1) {
-            const int msites = c.SP / ND;
-            for (int p = slot; p < c.P; p += G) {
-                const bool own = (p == slot) && on;
-#pragma unroll
-                for (int s = 0; s < 2; s++) {
-                    if (s >= msites) break;
-#pragma unroll
-                    for (int q = 0; q < ND; q++)
-                        Fc[c.exo + p * c.SP + s * ND + q] = (own && s < ns) ? cprd[s * ND + q] : 0.0;
-                }
-            }
         }
     }  // cell loop
 }
