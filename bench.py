@@ -84,14 +84,18 @@ def copy_probe(nbytes, reps=5):
 def measured_traffic(workload, key="fp64"):
     """HBM bytes per right-hand side of the timed calls (FETCH_SIZE +
     WRITE_SIZE over every kernel of one RHS, calibrated on a plain copy),
-    from the committed counter summary of this workload's --counters run,
-    or None."""
+    from the committed counter summary of this workload's --counters runs
+    (profiles/traffic_<workload>.md or profiles/k_rhs_<workload>.md), or
+    None."""
     import re
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
-    if not os.path.exists(path):
-        return None
-    m = re.search(rf"^traffic_bytes_per_rhs_{key}\s*=\s*([0-9.eE+]+)", open(path).read(), re.M)
-    return float(m.group(1)) if m else None
+    for name in (f"traffic_{workload}.md", f"k_rhs_{workload}.md"):
+        path = os.path.join(ROOT, "profiles", name)
+        if not os.path.exists(path):
+            continue
+        m = re.search(rf"^traffic_bytes_per_rhs_{key}\s*=\s*([0-9.eE+]+)", open(path).read(), re.M)
+        if m:
+            return float(m.group(1))
+    return None
 
 
 def host_threads():
