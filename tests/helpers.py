@@ -43,6 +43,17 @@ def ternary(res=(16, 16, 16), ngrains=4, nppt=2, radius=2, minlevel=2):
     return _cache[key]
 
 
+def with_interpolation(wl, kind):
+    """The same workload with the linear or quadratic interpolant
+    (src/utility.c:470-518) in place of the cubic one."""
+    key = ("interp", wl.name, kind)
+    if key not in _cache:
+        assert "interpolation: cubic\n" in wl.text
+        _cache[key] = W.Workload(wl.text.replace("interpolation: cubic\n", f"interpolation: {kind}\n"),
+                                 f"{wl.name}_{kind}")
+    return _cache[key]
+
+
 def initial_state(wl):
     X = oracle.setup_problem(wl.cfg, wl.mesh, wl.phase)
     if wl.layout["dp_size"]:

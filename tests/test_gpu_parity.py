@@ -57,6 +57,11 @@ WORKLOADS = {
 WORKLOADS["aniso3d_amr"] = lambda: H._cache.setdefault(
     ("aniso3d",), __import__("cipher_amd.workload", fromlist=["Workload"]).Workload(
         H.aniso_pair_yaml(res=(16, 16, 16), minlevel=2), "aniso3d"))
+# the linear and quadratic interpolants (src/utility.c:470-518) on 3D AMR
+# grain growth and on the 4-component configuration with every model
+for _k in ("linear", "quadratic"):
+    WORKLOADS[f"gg3d_amr_{_k}"] = (lambda k=_k: H.with_interpolation(H.gg3d_amr(), k))
+    WORKLOADS[f"chem4_{_k}"] = (lambda k=_k: H.with_interpolation(H.chem(), k))
 # the reference's example configurations (tests/golden, reduced grids)
 for _n in H.REF_CASES:
     WORKLOADS[f"ref_{_n}"] = (lambda n=_n: H.ref_case(n))
