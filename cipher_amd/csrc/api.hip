@@ -253,7 +253,8 @@ extern "C" int cipher_ctx_create(const cipher_config *cfg, int precision, cipher
         HIPCHK(hipMemcpy(c->tsv, nanv.data(), sizeof(double) * nanv.size(), hipMemcpyHostToDevice));
     }
     if (c->fold) d.tsv = c->tsv;
-    if (hipMalloc(&c->err, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->counts, (RHS_RING + 1) * RHS_COUNTERS * sizeof(int32_t)) != hipSuccess || hipMalloc(&c->partial, sizeof(double) * c->npart) != hipSuccess ||
+    if (hipMalloc(&c->d_cfg, sizeof(DevCfg)) != hipSuccess || cipher_sync_cfg(c) != CIPHER_OK ||
+        hipMalloc(&c->err, sizeof(int32_t)) != hipSuccess || hipMalloc(&c->counts, (RHS_RING + 1) * RHS_COUNTERS * sizeof(int32_t)) != hipSuccess || hipMalloc(&c->partial, sizeof(double) * c->npart) != hipSuccess ||
         hipMalloc(&c->norm, sizeof(double)) != hipSuccess ||
         hipHostMalloc(&c->h_norm, sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&c->h_err, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
@@ -261,6 +262,14 @@ extern "C" int cipher_ctx_create(const cipher_config *cfg, int precision, cipher
         return fail(CIPHER_ERR_HIP, "scratch allocation failed");
     }
     *out = c;
+    return CIPHER_OK;
+}
+
+int cipher_sync_cfg(cipher_ctx *c) {
+    if (!c->d_cfg) return CIPHER_OK;
+    if (c->stream) HIPCHK(hipStreamSynchronize(c->stream));
+    else HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(c->d_cfg, &c->dc, sizeof(DevCfg), hipMemcpyHostToDevice));
     return CIPHER_OK;
 }
 
@@ -278,6 +287,7 @@ extern "C" int cipher_ctx_destroy(cipher_ctx *c) {
     if (c->norm) (void)hipFree(c->norm);
     if (c->h_norm) (void)hipHostFree(c->h_norm);
     if (c->h_err) (void)hipHostFree(c->h_err);
+    if (c->d_cfg) (void)hipFree(c->d_cfg);
     delete c;
     return CIPHER_OK;
 }
@@ -308,7 +318,7 @@ extern "C" int cipher_set_temperature_rate(cipher_ctx *c, int has_rate, double r
     cipher_ts_invalidate(c);
     c->dc.has_trate = has_rate ? 1 : 0;
     c->dc.trate = rate;
-    return CIPHER_OK;
+    return cipher_sync_cfg(c);
 }
 
 template <int DIM>
@@ -449,6 +459,10 @@ extern "C" int cipher_upload_mesh(cipher_ctx *c, const cipher_mesh *m) {
         HIPCHK(hipMemcpy(dgw, gw.data(), sizeof(double) * gw.size(), hipMemcpyHostToDevice));
         c->dc.gw = dgw;
     }
+    if (int rs = cipher_sync_cfg(c)) {
+        free_mesh(c);
+        return rs;
+    }
     c->ncells = N;
     return CIPHER_OK;
 }
@@ -458,6 +472,7 @@ int cipher_rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active,
     if (!c->cell_off) return fail(CIPHER_ERR_STATE, "no mesh uploaded");
     RhsLaunch L;
     L.cfg = &c->dc;
+    L.dcfg = c->d_cfg;
     L.X = X;
     L.F = F;
     L.active = active;

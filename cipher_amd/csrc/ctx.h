@@ -74,6 +74,10 @@ struct cipher_ctx {
     int precision = CIPHER_FP64;
     cipher_layout L{};
     DevCfg dc{};
+    // a device copy of dc, which the list and prep kernels read through a
+    // pointer (a by-value kernel argument of ~50 fields is hoisted into
+    // scalar registers and spilled); refreshed whenever dc changes
+    DevCfg *d_cfg = nullptr;
     std::vector<void *> allocs;   // config tables
     // mesh
     long long ncells = 0;
@@ -106,6 +110,9 @@ struct cipher_ctx {
 
 // shared between the translation units of the C ABI
 int cipher_fail(int code, const std::string &msg);
+// copy c->dc to its device copy (after the context's stream has drained, so no
+// launch in flight reads a half-written configuration)
+int cipher_sync_cfg(cipher_ctx *c);
 // bins_mode: see RhsLaunch (0 for a right-hand side on its own)
 int cipher_rhs_impl(cipher_ctx *c, const double *X, double *F, uint32_t *active, int bins_mode = 0);
 int cipher_post_common(cipher_ctx *c, double *X, const int32_t *phase, bool setup);

@@ -71,6 +71,7 @@ struct ListRanges {
 
 struct RhsLaunch {
     const DevCfg *cfg;
+    const DevCfg *dcfg;       // device copy of *cfg (read by the list and prep kernels)
     const double *X;
     double *F;
     uint32_t *active;

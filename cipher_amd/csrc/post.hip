@@ -221,7 +221,7 @@ __global__ __launch_bounds__(256) void k_project_batch(const BatchMember *__rest
     const int mi = segmem[k];
     if (live && !live[mi]) return;
     const BatchMember &m = mem[mi];
-    const DevCfg c = m.c;
+    const DevCfg &c = m.c;
     project_cells<G>(X + m.xoff, m.glist, m.a.ncells, c, (long long)blockIdx.x - seg[k]);
 }
 
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(256) void k_union_batch(const BatchMember *__restri
     const int mi = segmem[k];
     if (live && !live[mi]) return;
     const BatchMember &m = mem[mi];
-    const DevCfg c = m.c;
+    const DevCfg &c = m.c;
     union_cells<G, NC>(X + m.xoff, m.glist, m.a.cell_off, m.a.adj, m.err, m.a.ncells, c,
                        (long long)blockIdx.x - seg[k]);
 }

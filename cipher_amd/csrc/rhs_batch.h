@@ -20,7 +20,7 @@ __global__ __launch_bounds__(256) void k_bin_batch(const BatchMember *__restrict
     const int mi = segmem[k];
     if (live && !live[mi]) return;
     const BatchMember &m = mem[mi];
-    const DevCfg c = m.c;
+    const DevCfg &c = m.c;
     bin_tile(X + m.xoff, m.a.ncells, c, nbins, m.bins, m.a.ncells, m.counts, (long long)blockIdx.x - seg[k]);
 }
 
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256, NC == 1 ? CIPHER_RHS_MINB1 : CIPHER_RHS_MINBN)
     a.redo_count = m.counts + 8 + q;
     a.cells = redo ? a.redo : m.bins + (long long)q * a.ncells;
     a.cells_count = redo ? a.redo_count : m.counts + q;
-    const DevCfg c = m.c;
+    const DevCfg &c = m.c;
     const long long blk = (long long)blockIdx.x - seg[k], nblk = seg[k + 1] - seg[k];
     switch (bl.lanes[q]) {
     case 1: rhs_cells<1, GMAX, NC, R>(a, c, blk, nblk); break;
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_prep_batch_lists(const BatchMember *__r
     const int mi = segmem[k];
     if (live && !live[mi]) return;
     const BatchMember &m = mem[mi];
-    const DevCfg c = m.c;
+    const DevCfg &c = m.c;
     if (!(NC > 1 || c.has_tcond)) return;
     const int q = segq[k];
     const long long N = m.a.ncells;

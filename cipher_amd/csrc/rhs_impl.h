@@ -1306,9 +1306,11 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
 // with that list's lane count.  One launch replaces a fork of concurrent
 // per-list launches and the cross-stream waits it needs.
 template <int GMAX, int NC, class R>
-__global__ __launch_bounds__(256, NC == 1 ? CIPHER_RHS_MINB1 : CIPHER_RHS_MINBN) void k_rhs_all(RhsArgs a, DevCfg c, const int32_t *bins,
-                                                                   int32_t *redo, int32_t *counts, ListRanges rg) {
+__global__ __launch_bounds__(256, NC == 1 ? CIPHER_RHS_MINB1 : CIPHER_RHS_MINBN) void k_rhs_all(RhsArgs a, const DevCfg *__restrict__ cfg,
+                                                                   const int32_t *bins, int32_t *redo, int32_t *counts,
+                                                                   ListRanges rg) {
     if (a.pred && *a.pred == 0) return;
+    const DevCfg &c = *cfg;
     int r = 0;
     while (r + 1 < rg.n && (int)blockIdx.x >= rg.start[r + 1]) r++;
     const int q = rg.list[r];
@@ -1333,9 +1335,11 @@ __global__ __launch_bounds__(256, NC == 1 ? CIPHER_RHS_MINB1 : CIPHER_RHS_MINBN)
 // as its active set needs.
 template <int GMAX, int NC>
 __global__ __launch_bounds__(256) void k_prep_all(const double *__restrict__ X, double *__restrict__ cm,
-                                                  long long ncells, DevCfg c, const int32_t *bins,
-                                                  const int32_t *counts, ListRanges rg, const int32_t *pred) {
+                                                  long long ncells, const DevCfg *__restrict__ cfg,
+                                                  const int32_t *bins, const int32_t *counts, ListRanges rg,
+                                                  const int32_t *pred) {
     if (pred && *pred == 0) return;
+    const DevCfg &c = *cfg;
     int r = 0;
     while (r + 1 < rg.n && (int)blockIdx.x >= rg.start[r + 1]) r++;
     const int q = rg.list[r];
@@ -1506,14 +1510,14 @@ cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st) {
         size_t l0 = 0;
         long long tp = 0;
         const ListRanges pr = make_ranges(ps, N, c, nbins, &l0, &tp);
-        hipLaunchKernelGGL((k_prep_all<G, NC>), dim3((unsigned)tp), dim3(256), 0, st, L.X, (double *)L.cm, N, c,
+        hipLaunchKernelGGL((k_prep_all<G, NC>), dim3((unsigned)tp), dim3(256), 0, st, L.X, (double *)L.cm, N, L.dcfg,
                            (const int32_t *)L.bins, (const int32_t *)L.counts, pr, L.pred);
     }
     size_t lds = 0;
     long long tot = 0;
     ListRanges rg = make_ranges(L.shape, N, c, nbins, &lds, &tot);
     if (precision == CIPHER_FP32_MIXED) {
-        hipLaunchKernelGGL((k_rhs_all<G, NC, float>), dim3((unsigned)tot), dim3(256), lds, st, a, c, L.bins, L.redo,
+        hipLaunchKernelGGL((k_rhs_all<G, NC, float>), dim3((unsigned)tot), dim3(256), lds, st, a, L.dcfg, L.bins, L.redo,
                            L.counts, rg);
         // FP64 recomputation of the cells whose flags sat in the FP32 band;
         // the recomputation lists are short, so each range gets at most
@@ -1527,10 +1531,10 @@ cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st) {
         long long tot2 = 0;
         ListRanges rr = make_ranges(rs, N, c, nbins, &lds2, &tot2);
         rr.redo = 1;
-        hipLaunchKernelGGL((k_rhs_all<G, NC, double>), dim3((unsigned)tot2), dim3(256), lds2, st, a, c, L.bins, L.redo,
+        hipLaunchKernelGGL((k_rhs_all<G, NC, double>), dim3((unsigned)tot2), dim3(256), lds2, st, a, L.dcfg, L.bins, L.redo,
                            L.counts, rr);
     } else {
-        hipLaunchKernelGGL((k_rhs_all<G, NC, double>), dim3((unsigned)tot), dim3(256), lds, st, a, c, L.bins, L.redo,
+        hipLaunchKernelGGL((k_rhs_all<G, NC, double>), dim3((unsigned)tot), dim3(256), lds, st, a, L.dcfg, L.bins, L.redo,
                            L.counts, rg);
     }
     return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
