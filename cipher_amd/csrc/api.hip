@@ -574,7 +574,7 @@ extern "C" int cipher_get_shape(const cipher_ctx *c, int32_t q, cipher_list_shap
 
 extern "C" int cipher_set_shape(cipher_ctx *c, int32_t q, const cipher_list_shape *shape) {
     if (!c || !shape || q < 0 || q >= 5) return fail(CIPHER_ERR_ARG, "bad argument");
-    if (!shape_valid(*shape, q, lanes_cap(c))) return fail(CIPHER_ERR_ARG, "list shape out of range");
+    if (!shape_valid(*shape, q, lanes_cap(c), c->dc.ncp)) return fail(CIPHER_ERR_ARG, "list shape out of range");
     ListShape &s = c->shape[q];
     s.lanes = shape->lanes;
     s.cap = shape->cap;

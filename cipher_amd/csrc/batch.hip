@@ -252,7 +252,7 @@ static int build_tables(cipher_batch *b) {
         for (int q = 0; q < 5; q++) {
             const int lanes = k.shape[q].lanes > 0 ? k.shape[q].lanes : (1 << q);
             L.shape.lanes[q] = lanes;
-            L.shape.map[q] = k.shape[q].map;
+            L.shape.map[q] = k.shape[q].map & 1;
         }
         for (int q = L.nbins - 1; q >= 0; q--) {
             order.push_back(q);
@@ -263,6 +263,21 @@ static int build_tables(cipher_batch *b) {
                  return list_blocks(b->m[i]->ncells, Nclass, 256 / L.shape.lanes[q], k.shape[q].cap);
              })))
             return rc;
+        // the single-phase list in a launch of its own: segments of lists
+        // 1.. and of list 0 (the merged table above still drives the prep
+        // records)
+        L.listsN = BatchSegs{};
+        L.lists0 = BatchSegs{};
+        if (own_launch(k.shape[0], k.ncp)) {
+            const std::vector<int> multi(order.begin(), order.end() - 1), single{0};
+            auto nb = [&](int i, int q) {
+                return list_blocks(b->m[i]->ncells, Nclass, 256 / L.shape.lanes[q], k.shape[q].cap);
+            };
+            if ((rc = make_list_segs(b, L.listsN, k.members, multi, nb))) return rc;
+            if ((rc = make_list_segs(b, L.lists0, k.members, single, nb))) return rc;
+            L.listsN.lds = L.lists.lds;
+            L.lists0.lds = L.lists.lds;
+        }
         L.relists.lds = L.lists.lds;
         if ((rc = make_list_segs(b, L.relists, k.members, order, [&](int i, int q) {
                  return list_blocks(b->m[i]->ncells, Nclass, 256 / L.shape.lanes[q],

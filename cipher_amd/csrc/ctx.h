@@ -18,14 +18,18 @@ struct ShapeStep {
 
 // Steps tried for list q of a mesh with lane capacity G: lanes 2^q..4*2^q
 // (single-phase cells without chemistry keep their one-lane path), both
-// block orders, caps of 16..65536 blocks (a cap below what the list needs
-// makes its blocks stride: slower, never wrong).
+// block orders, and for the single-phase list of a mesh with chemistry both
+// with and without a launch of its own (SHAPE_OWN), caps of 16..65536 blocks
+// (a cap below what the list needs makes its blocks stride: slower, never
+// wrong).
 static inline std::vector<ShapeStep> shape_steps(int q, int G, int ncp) {
     std::vector<ShapeStep> out;
     const int base = 1 << q;
     if (!(q == 0 && ncp == 1))
         for (int l = base; l <= std::min(G, base * 4); l *= 2) out.push_back({0, l});
     for (int m : {0, 1}) out.push_back({1, m});
+    if (q == 0 && ncp > 1)
+        for (int m : {cipher::SHAPE_OWN, cipher::SHAPE_OWN | 1}) out.push_back({1, m});
     for (int cap : {16, 128, 1024, 4096, 16384, 65536}) out.push_back({2, cap});
     return out;
 }
@@ -37,12 +41,14 @@ static inline cipher::ListShape apply_step(cipher::ListShape s, const ShapeStep 
     return s;
 }
 
-// a shape for list q of a mesh with lane capacity G
-static inline bool shape_valid(const cipher_list_shape &s, int q, int G) {
+// a shape for list q of a mesh with lane capacity G and ncp components
+static inline bool shape_valid(const cipher_list_shape &s, int q, int G, int ncp) {
     const int base = 1 << q;
     const bool lanes_ok = s.lanes == 0 || ((s.lanes & (s.lanes - 1)) == 0 && s.lanes >= base && s.lanes <= G &&
                                            s.lanes <= 16);
-    return lanes_ok && s.cap >= 1 && (s.map == 0 || s.map == 1);
+    const bool map_ok = s.map == 0 || s.map == 1 ||
+                        (q == 0 && ncp > 1 && (s.map == cipher::SHAPE_OWN || s.map == (cipher::SHAPE_OWN | 1)));
+    return lanes_ok && s.cap >= 1 && map_ok;
 }
 
 struct TsState {

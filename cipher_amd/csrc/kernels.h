@@ -30,11 +30,20 @@ struct RhsArgs {
 // Launch shape of one per-size cell list: lanes per cell (at least the
 // list's active-set bound; more lanes take more faces per step), grid cap
 // and block order.  Shapes change timing only, never results.
+// map: bit 0 the block order (RhsArgs::map); SHAPE_OWN (list 0 of a mesh
+// with chemistry, one lane per cell): the single-phase cells run in a launch
+// of their own, compiled for more resident waves than the merged launch of
+// the multi-phase lists allows
+constexpr int SHAPE_OWN = 2;
 struct ListShape {
     int lanes = 0;            // 0: 2^q for list q
     int cap = 16384;          // blocks of the list's range
-    int map = 0;              // RhsArgs::map
+    int map = 0;              // block order (bit 0) and SHAPE_OWN
 };
+// the single-phase list runs in its own launch
+static inline bool own_launch(const ListShape &s0, int ncp) {
+    return ncp > 1 && (s0.map & SHAPE_OWN) && s0.lanes <= 1;
+}
 
 // per-mesh list counters: 0..4 the per-size lists, 8..12 their FP32-mixed
 // recomputation lists
@@ -175,6 +184,9 @@ struct BatchLaunch {
     int do_bin;               // sort the cells into lists (0: reuse the previous call's lists)
     BatchSegs fold, prep, bin, lists, post, dofs;
     BatchSegs relists;        // the list launch's segments with at most RHS_REDO_CAP blocks per list
+    // with the single-phase list in a launch of its own (own_launch): the
+    // list launch's segments of lists 1.. and of list 0 (else empty)
+    BatchSegs listsN, lists0;
     BatchLists shape;
     const double *X;
     double *F;

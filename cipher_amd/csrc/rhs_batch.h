@@ -60,6 +60,32 @@ __global__ __launch_bounds__(256, NC == 1 ? CIPHER_RHS_MINB1 : CIPHER_RHS_MINBN)
     }
 }
 
+// The single-phase lists (list 0) of all members of a chemistry class in a
+// launch of their own, compiled for more resident waves (k_rhs_one).
+template <int GMAX, int NC, class R>
+__global__ __launch_bounds__(256, CIPHER_RHS_MINB_ONE(NC)) void k_rhs_batch_one(const BatchMember *__restrict__ mem,
+                                                                            const int32_t *__restrict__ live,
+                                                                            const int32_t *__restrict__ seg,
+                                                                            const int32_t *__restrict__ segmem,
+                                                                            int nseg, int map, const double *X,
+                                                                            double *F, uint32_t *active) {
+    const int k = seg_find(seg, nseg, blockIdx.x);
+    const int mi = segmem[k];
+    if (live && !live[mi]) return;
+    const BatchMember &m = mem[mi];
+    RhsArgs a = m.a;
+    a.X = X + m.xoff;
+    a.F = F + m.xoff;
+    a.active = active ? active + m.celloff : nullptr;
+    a.map = map;
+    a.redo = m.a.redo;
+    a.redo_count = m.counts + 8;
+    a.cells = m.bins;
+    a.cells_count = m.counts;
+    const long long blk = (long long)blockIdx.x - seg[k], nblk = seg[k + 1] - seg[k];
+    rhs_cells<1, GMAX, NC, R>(a, m.c, blk, nblk);
+}
+
 // the prep records of all lists of all members (the list launch's
 // segments), each cell with its list's lane count
 template <int GMAX, int NC>
@@ -100,16 +126,35 @@ cipher_status batch_G(const BatchLaunch &L, int precision, hipStream_t st) {
     if (L.prep.nblocks && s.nblocks)
         hipLaunchKernelGGL((k_prep_batch_lists<G, NC>), dim3((unsigned)s.nblocks), dim3(256), 0, st, L.mem, L.live,
                            s.seg, s.segmem, s.segq, s.nseg, L.shape, L.X);
+    // with the single-phase lists in a launch of their own, the merged
+    // launch covers lists 1.. (then list 0 follows on the same stream)
+    const bool own0 = NC > 1 && L.lists0.nblocks > 0;
+    const BatchSegs &sm = own0 ? L.listsN : s;
+    const BatchSegs &s0 = L.lists0;
     if (s.nblocks) {
         if (precision == CIPHER_FP32_MIXED) {
-            hipLaunchKernelGGL((k_rhs_batch<G, NC, float>), dim3((unsigned)s.nblocks), dim3(256), s.lds, st, L.mem,
-                               L.live, s.seg, s.segmem, s.segq, s.nseg, L.shape, 0, L.X, L.F, L.active);
+            if (sm.nblocks)
+                hipLaunchKernelGGL((k_rhs_batch<G, NC, float>), dim3((unsigned)sm.nblocks), dim3(256), sm.lds, st,
+                                   L.mem, L.live, sm.seg, sm.segmem, sm.segq, sm.nseg, L.shape, 0, L.X, L.F, L.active);
+            if constexpr (NC > 1) {
+                if (own0)
+                    hipLaunchKernelGGL((k_rhs_batch_one<G, NC, float>), dim3((unsigned)s0.nblocks), dim3(256), s0.lds,
+                                       st, L.mem, L.live, s0.seg, s0.segmem, s0.nseg, L.shape.map[0], L.X, L.F,
+                                       L.active);
+            }
             const BatchSegs &r = L.relists;
             hipLaunchKernelGGL((k_rhs_batch<G, NC, double>), dim3((unsigned)r.nblocks), dim3(256), r.lds, st, L.mem,
                                L.live, r.seg, r.segmem, r.segq, r.nseg, L.shape, 1, L.X, L.F, L.active);
         } else {
-            hipLaunchKernelGGL((k_rhs_batch<G, NC, double>), dim3((unsigned)s.nblocks), dim3(256), s.lds, st, L.mem,
-                               L.live, s.seg, s.segmem, s.segq, s.nseg, L.shape, 0, L.X, L.F, L.active);
+            if (sm.nblocks)
+                hipLaunchKernelGGL((k_rhs_batch<G, NC, double>), dim3((unsigned)sm.nblocks), dim3(256), sm.lds, st,
+                                   L.mem, L.live, sm.seg, sm.segmem, sm.segq, sm.nseg, L.shape, 0, L.X, L.F, L.active);
+            if constexpr (NC > 1) {
+                if (own0)
+                    hipLaunchKernelGGL((k_rhs_batch_one<G, NC, double>), dim3((unsigned)s0.nblocks), dim3(256), s0.lds,
+                                       st, L.mem, L.live, s0.seg, s0.segmem, s0.nseg, L.shape.map[0], L.X, L.F,
+                                       L.active);
+            }
         }
     }
     return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;

@@ -42,6 +42,11 @@
 #ifndef CIPHER_RHS_MINBN
 #define CIPHER_RHS_MINBN 2
 #endif
+// the same for the launch of the single-phase cells of a mesh with
+// chemistry (one lane per cell, no pair loops): without spills, 2 components
+// fit 128 VGPRs (4 waves per SIMD) and 3 components 168 (3 waves); 4 and 5
+// components take 200-240 (2 waves, where the merged launch spills 70-150)
+#define CIPHER_RHS_MINB_ONE(NC) ((NC) <= 2 ? 4 : ((NC) == 3 ? 3 : 2))
 
 namespace cipher {
 
@@ -1330,6 +1335,23 @@ __global__ __launch_bounds__(256, NC == 1 ? CIPHER_RHS_MINB1 : CIPHER_RHS_MINBN)
     }
 }
 
+// The single-phase list (list 0, one lane per cell) of a mesh with
+// chemistry in a launch of its own: its body has no pair loops and needs
+// fewer registers than the merged launch, whose register count is set by the
+// multi-phase paths, so more of its waves are resident.
+template <int GMAX, int NC, class R>
+__global__ __launch_bounds__(256, CIPHER_RHS_MINB_ONE(NC)) void k_rhs_one(RhsArgs a, const DevCfg *__restrict__ cfg,
+                                                                      const int32_t *bins, int32_t *redo,
+                                                                      int32_t *counts, int map) {
+    if (a.pred && *a.pred == 0) return;
+    a.redo = redo;
+    a.redo_count = counts + 8;
+    a.cells = bins;
+    a.cells_count = counts;
+    a.map = map;
+    rhs_cells<1, GMAX, NC, R>(a, *cfg, blockIdx.x, gridDim.x);
+}
+
 // The prep records of all per-size lists in one launch (block ranges as in
 // k_rhs_all, 2^q lanes per cell of list q), so a cell occupies as many lanes
 // as its active set needs.
@@ -1439,30 +1461,38 @@ __global__ void k_fill_i32(int32_t *p, int n, int32_t v) {
 }
 
 
-// Ranges of the merged launch for lists 0..nbins-1 (launch order: largest
+// blocks of list q's range: enough for every cell, at most its cap
+static inline long long list_range_blocks(const ListShape &sh, int q, long long N) {
+    const int lanes = sh.lanes > 0 ? sh.lanes : (1 << q);
+    const int cpb = 256 / lanes;
+    long long nb = (N + cpb - 1) / cpb;
+    if (nb > sh.cap) nb = sh.cap;
+    if (nb < 1) nb = 1;
+    return nb;
+}
+
+// Ranges of the merged launch for lists qlo..nbins-1 (launch order: largest
 // active sets first); *lds: the largest LDS need over the ranges.
 static inline ListRanges make_ranges(const ListShape *shape, long long N, const DevCfg &c, int nbins, size_t *lds,
-                                     long long *nblocks) {
+                                     long long *nblocks, int qlo = 0) {
     ListRanges rg{};
-    rg.n = nbins;
+    rg.n = nbins - qlo;
     long long tot = 0;
     size_t l = 0;
-    for (int r = 0; r < nbins; r++) {
+    for (int r = 0; r < nbins - qlo; r++) {
         const int q = nbins - 1 - r;
         const ListShape &sh = shape[q];
         const int lanes = sh.lanes > 0 ? sh.lanes : (1 << q);
         const int cpb = 256 / lanes;
-        long long nb = (N + cpb - 1) / cpb;
-        if (nb > sh.cap) nb = sh.cap;
-        if (nb < 1) nb = 1;
+        const long long nb = list_range_blocks(sh, q, N);
         rg.start[r] = (int32_t)tot;
         rg.list[r] = q;
         rg.lanes[r] = lanes;
-        rg.map[r] = sh.map;
+        rg.map[r] = sh.map & 1;
         tot += nb;
         l = std::max(l, rhs_lds_bytes(c, lanes, cpb));
     }
-    rg.start[nbins] = (int32_t)tot;
+    rg.start[nbins - qlo] = (int32_t)tot;
     *lds = l;
     *nblocks = tot;
     return rg;
@@ -1515,10 +1545,20 @@ cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st) {
     }
     size_t lds = 0;
     long long tot = 0;
-    ListRanges rg = make_ranges(L.shape, N, c, nbins, &lds, &tot);
+    // the single-phase list in a launch of its own (after the merged launch
+    // of the multi-phase lists, on the same stream)
+    const bool own0 = own_launch(L.shape[0], NC);
+    ListRanges rg = make_ranges(L.shape, N, c, nbins, &lds, &tot, own0 ? 1 : 0);
+    const long long nb0 = own0 ? list_range_blocks(L.shape[0], 0, N) : 0;
+    const size_t lds0 = rhs_lds_bytes(c, 1, 256);
     if (precision == CIPHER_FP32_MIXED) {
         hipLaunchKernelGGL((k_rhs_all<G, NC, float>), dim3((unsigned)tot), dim3(256), lds, st, a, L.dcfg, L.bins, L.redo,
                            L.counts, rg);
+        if constexpr (NC > 1) {
+            if (own0)
+                hipLaunchKernelGGL((k_rhs_one<G, NC, float>), dim3((unsigned)nb0), dim3(256), lds0, st, a, L.dcfg,
+                                   L.bins, L.redo, L.counts, L.shape[0].map & 1);
+        }
         // FP64 recomputation of the cells whose flags sat in the FP32 band;
         // the recomputation lists are short, so each range gets at most
         // RHS_REDO_CAP blocks (they stride over longer lists)
@@ -1536,6 +1576,11 @@ cipher_status launch_G(const RhsLaunch &L, int precision, hipStream_t st) {
     } else {
         hipLaunchKernelGGL((k_rhs_all<G, NC, double>), dim3((unsigned)tot), dim3(256), lds, st, a, L.dcfg, L.bins, L.redo,
                            L.counts, rg);
+        if constexpr (NC > 1) {
+            if (own0)
+                hipLaunchKernelGGL((k_rhs_one<G, NC, double>), dim3((unsigned)nb0), dim3(256), lds0, st, a, L.dcfg,
+                                   L.bins, L.redo, L.counts, L.shape[0].map & 1);
+        }
     }
     return hipGetLastError() == hipSuccess ? CIPHER_OK : CIPHER_ERR_HIP;
 }

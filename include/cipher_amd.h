@@ -380,9 +380,12 @@ int cipher_output(cipher_ctx *ctx, const double *X_dev, int32_t nfields, const c
  * most 2^q active phases; all lists run in one launch of 256-thread blocks,
  * one block range per list): lanes per cell (0 = 2^q; else a power of two
  * between 2^q and the capacity's lane count, at most 16), the list's block
- * cap (its range strides beyond it), and the block order (0 as launched, 1
- * one contiguous part of the list per XCD).  Shapes change timing only,
- * never results.  No counterpart in the reference. */
+ * cap (its range strides beyond it), and map: bit 0 the block order (0 as
+ * launched, 1 one contiguous part of the list per XCD), bit 1 (value 2; list
+ * 0 of a configuration with chemistry, one lane per cell) the single-phase
+ * cells in a launch of their own after the merged launch of the other lists,
+ * compiled for more resident waves.  Shapes change timing only, never
+ * results.  No counterpart in the reference. */
 typedef struct cipher_list_shape {
     int32_t lanes, cap, map, pad_;
 } cipher_list_shape;

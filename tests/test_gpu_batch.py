@@ -190,7 +190,9 @@ def test_batch_autotune_keeps_results():
     print(f"[batch-tune] {ms:.4f} ms, shapes {shapes}", flush=True)
     assert ms > 0.0 and shapes
     for ncp, capl, q, lanes, cap, mp in shapes:
-        assert (1 << q) <= lanes <= capl and mp in (0, 1) and cap >= 1
+        # map 2 and 3: the single-phase list of a chemistry class in a launch of its own
+        assert (1 << q) <= lanes <= capl and cap >= 1
+        assert mp in (0, 1) or (q == 0 and ncp > 1 and mp in (2, 3))
     assert np.array_equal(_host(b.rhs(X)), F0)
     b.close()
 

@@ -19,8 +19,10 @@ def _run(ctx, X):
 
 
 SHAPES = [
-    # (list, lanes, cap, map)
+    # (list, lanes, cap, map); map 2 and 3: the single-phase list of a mesh
+    # with chemistry in a launch of its own, in launch order and per-XCD order
     (1, 4, 16384, 0), (1, 8, 4096, 1), (2, 8, 65536, 1), (0, 1, 16, 1), (3, 8, 1, 1), (2, 4, 3, 0),
+    (0, 1, 16384, 2), (0, 1, 64, 3), (0, 1, 16384, 0),
 ]
 
 
@@ -44,8 +46,11 @@ def test_every_shape_gives_the_same_residual(name, precision):
     print(f"[shapes] {name} precision {precision} default {ctx.shapes()}", flush=True)
     P = wl.layout["pf_size"]
     capl = 4 if P <= 4 else (8 if P <= 8 else 16)
+    chem = bool(wl.layout["dp_size"])
     for q, lanes, cap, mp in SHAPES:
-        if q == 0 and wl.layout["dp_size"]:
+        if mp & 2 and not chem:
+            continue       # a launch of its own is for chemistry meshes only
+        if q == 0 and chem and mp < 2 and cap != 16384:
             lanes = 2      # chemistry cells of list 0 may take more lanes
         if lanes > capl:
             continue
@@ -65,7 +70,7 @@ def test_set_shape_rejects_bad_shapes():
     ctx = Context(wl.cfg)
     ctx.upload_mesh(wl.mesh)
     for q, lanes, cap, mp in [(1, 1, 16, 0), (2, 16, 16, 0), (0, 3, 16, 0), (0, 1, 0, 0), (0, 1, 16, 2),
-                              (5, 1, 16, 0)]:
+                              (5, 1, 16, 0), (1, 2, 16, 2), (0, 1, 16, 4)]:
         with pytest.raises(abi.CipherError) as e:
             ctx.set_shape(q, lanes, cap, mp)
         assert e.value.code == abi.ERR_ARG
