@@ -107,7 +107,7 @@ class Mesh(C.Structure):
 class TsParams(C.Structure):
     _fields_ = [("atol", C.c_double), ("rtol", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
                 ("safety", C.c_double), ("reject_safety", C.c_double), ("clip_lo", C.c_double),
-                ("clip_hi", C.c_double), ("use_graph", C.c_int32), ("pad_", C.c_int32)]
+                ("clip_hi", C.c_double), ("use_graph", C.c_int32), ("method", C.c_int32)]
 
 
 class TsCtrl(C.Structure):
@@ -131,8 +131,23 @@ class ListShape(C.Structure):
         return (self.lanes, self.cap, self.map)
 
 
-def ts_params(atol, rtol, dt_min, dt_max, safety=0.9, reject_safety=0.5, clip=(0.1, 10.0), use_graph=False):
-    return TsParams(atol, rtol, dt_min, dt_max, safety, reject_safety, clip[0], clip[1], 1 if use_graph else 0, 0)
+# TSRK types of cipher_ts_params.method, by petscoptions' -ts_rk_type name
+RK_3BS, RK_2A = 0, 1
+RK_METHODS = {"3bs": RK_3BS, "2a": RK_2A}
+
+
+def rk_method(name):
+    """cipher_ts_params.method of a '-ts_rk_type' name; ERR_UNSUPPORTED for
+    a type the device integrator does not run."""
+    if name not in RK_METHODS:
+        raise CipherError(ERR_UNSUPPORTED, f"petscoptions -ts_rk_type {name} (3bs and 2a are integrated)")
+    return RK_METHODS[name]
+
+
+def ts_params(atol, rtol, dt_min, dt_max, safety=0.9, reject_safety=0.5, clip=(0.1, 10.0), use_graph=False,
+              method=RK_3BS):
+    return TsParams(atol, rtol, dt_min, dt_max, safety, reject_safety, clip[0], clip[1], 1 if use_graph else 0,
+                    method)
 
 
 _HERE = os.path.dirname(os.path.abspath(__file__))

@@ -42,6 +42,9 @@ static const double A2[1] = {1.0 / 2.0};
 static const double A3[1] = {3.0 / 4.0};
 static const double B[3] = {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0};
 static const double BH[4] = {7.0 / 24.0, 1.0 / 4.0, 1.0 / 3.0, 1.0 / 8.0};
+// Heun's method with the explicit Euler step as its embedded solution
+static const double H2A[1] = {1.0};
+static const double H2B[2] = {1.0 / 2.0, 1.0 / 2.0};
 
 namespace {
 
@@ -749,7 +752,7 @@ extern "C" int cipher_batch_ts_setup(cipher_batch *b, const cipher_ts_params *p)
     if (!b || !p) return cipher_fail(CIPHER_ERR_ARG, "null argument");
     if (!(p->atol >= 0.0) || !(p->rtol >= 0.0) || !(p->atol > 0.0 || p->rtol > 0.0) || !(p->dt_min > 0.0) ||
         !(p->dt_max >= p->dt_min) || !(p->safety > 0.0) || !(p->reject_safety > 0.0) || !(p->clip_lo > 0.0) ||
-        !(p->clip_hi >= p->clip_lo))
+        !(p->clip_hi >= p->clip_lo) || (p->method != CIPHER_RK_3BS && p->method != CIPHER_RK_2A))
         return cipher_fail(CIPHER_ERR_ARG, "integrator parameters out of range");
     ts_free(b);
     BatchTs &s = b->ts;
@@ -781,6 +784,16 @@ static int enqueue_attempt(cipher_batch *b, const double *X) {
     // stages 2-4 keep the active sets of X: bin once per attempt
     if ((rc = batch_rhs_impl(b, X, s.K[0], nullptr, live))) return rc;
     A.live = live;
+    if (s.p.method == CIPHER_RK_2A) {
+        const double *k12[2] = {s.K[0], s.K[1]};
+        if (launch_axpy_dt_batch(A, s.Xh, X, s.dt_dev, 1, H2A, k1, st)) return cipher_fail(CIPHER_ERR_HIP, "Euler step");
+        if ((rc = batch_rhs_impl(b, s.Xh, s.K[1], nullptr, live, true))) return rc;
+        if (launch_axpy_dt_batch(A, s.Xn, X, s.dt_dev, 2, H2B, k12, st)) return cipher_fail(CIPHER_ERR_HIP, "Heun step");
+        A.live = live;
+        if (launch_wrms_batch(A, s.Xn, s.Xh, s.p.atol, s.p.rtol, b->d_norms, st))
+            return cipher_fail(CIPHER_ERR_HIP, "norm");
+        return CIPHER_OK;
+    }
     if (launch_axpy_dt_batch(A, s.Y, X, s.dt_dev, 1, A2, k1, st)) return cipher_fail(CIPHER_ERR_HIP, "stage 2");
     if ((rc = batch_rhs_impl(b, s.Y, s.K[1], nullptr, live, true))) return rc;
     if (launch_axpy_dt_batch(A, s.Y, X, s.dt_dev, 1, A3, k2, st)) return cipher_fail(CIPHER_ERR_HIP, "stage 3");

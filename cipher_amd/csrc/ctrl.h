@@ -78,9 +78,10 @@ __host__ __device__ inline double ctl_atan(double x) {
 
 // Acceptance and next step size for an attempt of size dt with error norm
 // err (PETSc's TSAdapt: accept if err <= 1 or dt is at the minimum; a PI
-// controller, Soederlind's PI.4.2 with k*beta1 = 3/5, k*beta2 = -1/5, k = 3
-// for the second-order error estimate, the elementary controller (exponent
-// 1/k) without history, and the arctan limiter rho = 1 + atan(r - 1);
+// controller, Soederlind's PI.4.2 with k*beta1 = 3/5, k*beta2 = -1/5, k the
+// order of the method (3 for Bogacki-Shampine 3(2), 2 for Heun 2(1)), the
+// elementary controller (exponent 1/k) without history, and the arctan
+// limiter rho = 1 + atan(r - 1);
 // safety factor, extra factor after consecutive rejections, ratio clip and
 // step limits).  Returns CIPHER_ERR_NONFINITE for a non-finite err at the
 // minimum step (the step is then shrunk as far as allowed).
@@ -98,7 +99,7 @@ __host__ __device__ inline int ctl_adapt(const cipher_ts_params &p, cipher_ts_ct
     }
     const bool acc = err <= 1.0 || at_min;
     if (!acc && st.rejected_last) safety *= p.reject_safety;
-    const double k = 3.0;
+    const double k = p.method == CIPHER_RK_2A ? 2.0 : 3.0;
     double rho;
     if (err > 0.0) {
         double r = ctl_pow(1.0 / err, (st.err_prev > 0.0 ? 0.6 : 1.0) / k);
@@ -169,7 +170,7 @@ __device__ inline void ts_decide(TsDev &s, const cipher_ts_params &p, double err
     *acc = 0;
     if (!s.active) return;
     s.err = err;
-    s.rhs_evals += 4;
+    s.rhs_evals += p.method == CIPHER_RK_2A ? 2 : 4;
     int a = 0;
     double dtn = s.dt_try;
     const int rc = ctl_adapt(p, s.ctrl, s.dt_try, err, &a, &dtn);

@@ -10,6 +10,8 @@
 //                           K3 = f(X + 3dt/4 K2)
 //                           Xn = X + dt (2/9 K1 + 1/3 K2 + 4/9 K3),  K4 = f(Xn)
 //                           Xh = X + dt (7/24 K1 + 1/4 K2 + 1/3 K3 + 1/8 K4)
+//   Heun 2(1) ("2a"):       K1 = f(X),  Xh = X + dt K1
+//                           K2 = f(Xh), Xn = X + dt (1/2 K1 + 1/2 K2)
 //   error  = WRMS(Xn, Xh) with tol_i = atol + rtol max(|Xn_i|, |Xh_i|)
 //   accept = error <= 1 (or dt at dt_min), then X = Xn and PostStep.
 //
@@ -48,6 +50,9 @@ static const double A2[1] = {1.0 / 2.0};
 static const double A3[1] = {3.0 / 4.0};
 static const double B[3] = {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0};
 static const double BH[4] = {7.0 / 24.0, 1.0 / 4.0, 1.0 / 3.0, 1.0 / 8.0};
+// Heun's method with the explicit Euler step as its embedded solution
+static const double H2A[1] = {1.0};
+static const double H2B[2] = {1.0 / 2.0, 1.0 / 2.0};
 
 void cipher_ts_invalidate(cipher_ctx *c);
 
@@ -101,7 +106,7 @@ extern "C" int cipher_ts_setup(cipher_ctx *c, const cipher_ts_params *p) {
     if (!c->cell_off) return cipher_fail(CIPHER_ERR_STATE, "no mesh uploaded");
     if (!(p->atol >= 0.0) || !(p->rtol >= 0.0) || !(p->atol > 0.0 || p->rtol > 0.0) || !(p->dt_min > 0.0) ||
         !(p->dt_max >= p->dt_min) || !(p->safety > 0.0) || !(p->reject_safety > 0.0) || !(p->clip_lo > 0.0) ||
-        !(p->clip_hi >= p->clip_lo))
+        !(p->clip_hi >= p->clip_lo) || (p->method != CIPHER_RK_3BS && p->method != CIPHER_RK_2A))
         return cipher_fail(CIPHER_ERR_ARG, "integrator parameters out of range");
     cipher_ts_free(c);
     TsState &s = c->ts;
@@ -136,6 +141,15 @@ static int enqueue_attempt(cipher_ctx *c, const double *X) {
     // the stage values keep X's active sets (the rate of an AS entry is 0),
     // so the cells are binned once per attempt and the lists reused
     if ((rc = cipher_rhs_impl(c, X, s.K[0], nullptr, 1))) return rc;
+    if (s.p.method == CIPHER_RK_2A) {
+        const double *k12[2] = {s.K[0], s.K[1]};
+        if (launch_axpy_dt(n, s.Xh, X, s.dt_dev, 1, H2A, k1, st)) return cipher_fail(CIPHER_ERR_HIP, "Euler step");
+        if ((rc = cipher_rhs_impl(c, s.Xh, s.K[1], nullptr, 2))) return rc;
+        if (launch_axpy_dt(n, s.Xn, X, s.dt_dev, 2, H2B, k12, st)) return cipher_fail(CIPHER_ERR_HIP, "Heun step");
+        if (launch_wrms(s.Xn, s.Xh, n, s.p.atol, s.p.rtol, c->partial, wrms_parts(n), c->norm, st))
+            return cipher_fail(CIPHER_ERR_HIP, "norm");
+        return CIPHER_OK;
+    }
     if (launch_axpy_dt(n, s.Y, X, s.dt_dev, 1, A2, k1, st)) return cipher_fail(CIPHER_ERR_HIP, "stage 2");
     if ((rc = cipher_rhs_impl(c, s.Y, s.K[1], nullptr, 2))) return rc;
     if (launch_axpy_dt(n, s.Y, X, s.dt_dev, 1, A3, k2, st)) return cipher_fail(CIPHER_ERR_HIP, "stage 3");

@@ -84,13 +84,15 @@ def solve(ctx, cfg, forest, X, params, t0=0.0, times=None, rates=(), amrinterval
     and state.  times / rates / amrinterval default to the configuration's.
     With output_dir, chunks also end at every outputfreq-th step, where the
     output step writes its VTU file (before a remesh at the same step, as
-    PostStep runs inside the solve); Run.outputs lists the files."""
-    if getattr(cfg, "ts_rk_type", "3bs") != "3bs":
-        # the device integrator is Bogacki-Shampine 3(2); integrating a
-        # configuration written for another scheme would change its
-        # trajectory and step counts without notice
-        raise abi.CipherError(abi.ERR_UNSUPPORTED, f"petscoptions -ts_rk_type {cfg.ts_rk_type} "
-                              "(only 3bs is integrated)")
+    PostStep runs inside the solve); Run.outputs lists the files.  The
+    configuration's TSRK type (petscoptions -ts_rk_type, 3bs by default)
+    must be params.method (abi.rk_method): integrating a configuration with
+    another scheme would change its trajectory and step counts without
+    notice."""
+    method = abi.rk_method(getattr(cfg, "ts_rk_type", "3bs"))
+    if params is None or params.method != method:
+        raise abi.CipherError(abi.ERR_ARG, f"integrator method {getattr(params, 'method', None)} does not match "
+                              f"petscoptions -ts_rk_type {getattr(cfg, 'ts_rk_type', '3bs')} (method {method})")
     times = list(cfg.times if times is None else times)
     if rates == ():
         rates = cfg.temperature_rates

@@ -285,8 +285,14 @@ typedef struct cipher_ts_params {
     double reject_safety;       /* extra factor after consecutive rejections (0.5) */
     double clip_lo, clip_hi;    /* bounds of the step ratio (0.1, 10) */
     int32_t use_graph;          /* replay each attempt from a captured HIP graph */
-    int32_t pad_;
+    int32_t method;             /* CIPHER_RK_3BS (0) or CIPHER_RK_2A: TSRK type, src/cipher.c:1104
+                                   (petscoptions -ts_rk_type) */
 } cipher_ts_params;
+
+/* TSRK types (PETSc's names): Bogacki-Shampine 3(2), four stages with the
+ * last one evaluated at the new solution; Heun's method 2(1), two stages,
+ * the error estimate against the explicit Euler step of the first stage. */
+enum { CIPHER_RK_3BS = 0, CIPHER_RK_2A = 1 };
 
 /* Controller memory carried between steps (zero-initialise before a run). */
 typedef struct cipher_ts_ctrl {

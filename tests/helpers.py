@@ -293,12 +293,20 @@ BS3_B = (2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0)
 BS3_BH = (7.0 / 24.0, 1.0 / 4.0, 1.0 / 3.0, 1.0 / 8.0)
 
 
-def cpu_attempt(wl, X, dt, atol, rtol, rhs=None):
-    """One Bogacki-Shampine 3(2) attempt with the oracle RHS, in the stage
-    arithmetic of the device integrator: Y = X + ((a0 K0 + a1 K1) + ...),
-    a_j = dt * A_j.  Returns (new solution, embedded solution, error norm)."""
+def cpu_attempt(wl, X, dt, atol, rtol, rhs=None, method=0):
+    """One Bogacki-Shampine 3(2) attempt (method 0, abi.RK_3BS) or Heun 2(1)
+    attempt (method 1, abi.RK_2A: Xh = X + dt K1 the explicit Euler step,
+    Xn = X + dt (K1 + K2)/2 with K2 = f(Xh)) with the oracle RHS, in the
+    stage arithmetic of the device integrator: Y = X + ((a0 K0 + a1 K1) +
+    ...), a_j = dt * A_j.  Returns (new solution, embedded solution, error
+    norm)."""
     f = rhs or (lambda Y: oracle.rhs(wl.cfg, wl.mesh, Y))
     K1 = f(X)
+    if method == 1:
+        Xh = X + (dt * 1.0) * K1
+        K2 = f(Xh)
+        Xn = X + ((dt * 0.5) * K1 + (dt * 0.5) * K2)
+        return Xn, Xh, oracle.wrms_norm(Xn, Xh, atol, rtol)
     K2 = f(X + (dt * BS3_A2) * K1)
     K3 = f(X + (dt * BS3_A3) * K2)
     a = [dt * b for b in BS3_B]
@@ -322,7 +330,7 @@ def cpu_solve(wl, X, t0, t_end, dt, params, max_steps=0, ctrl=None):
         dt_try = t_end - t if t + dt > t_end else dt
         if not dt_try > 0:
             break
-        Xn, _, err = cpu_attempt(wl, X, dt_try, params.atol, params.rtol)
+        Xn, _, err = cpu_attempt(wl, X, dt_try, params.atol, params.rtol, method=params.method)
         acc, dt_next = ts_adapt(params, ctrl, dt_try, err)
         dts.append((dt_try, err, acc))
         if acc:

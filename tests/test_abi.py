@@ -53,7 +53,8 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
               ("cipher_material", "model", abi.Material), ("cipher_material", "source", abi.Material),
               ("cipher_interface", "e_act", abi.Interface), ("cipher_bc", "boundary_id", abi.BC),
               ("cipher_mesh", "ncells", abi.Mesh), ("cipher_tseries", "exp", abi.TSeries),
-              ("cipher_ts_params", "use_graph", abi.TsParams), ("cipher_ts_stats", "t", abi.TsStats),
+              ("cipher_ts_params", "use_graph", abi.TsParams), ("cipher_ts_params", "method", abi.TsParams),
+              ("cipher_ts_stats", "t", abi.TsStats),
               ("cipher_list_shape", "map", abi.ListShape)]
     src = tmp_path / "sz.c"
     body = "".join(f'printf("%zu\\n", sizeof({n}));' for n in names)

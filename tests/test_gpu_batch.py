@@ -144,8 +144,9 @@ def test_batch_poststep_names_the_overflowing_member():
     b.close()
 
 
-@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_MIXED], ids=["fp64", "fp32mixed"])
-def test_batch_ts_solve_equals_members_alone(precision):
+@pytest.mark.parametrize("precision,method", [(abi.FP64, abi.RK_3BS), (abi.FP32_MIXED, abi.RK_3BS),
+                                              (abi.FP64, abi.RK_2A)], ids=["fp64", "fp32mixed", "fp64-2a"])
+def test_batch_ts_solve_equals_members_alone(precision, method):
     import torch
     from cipher_amd.runtime import Batch
     wls = H.ensemble()
@@ -157,7 +158,7 @@ def test_batch_ts_solve_equals_members_alone(precision):
         scale = 1.0 if wl.layout["dp_size"] == 0 else 1e-9
         t_end.append(scale * (0.6 + 0.2 * i))
         dt0.append(scale * 0.05)
-    params = abi.ts_params(1e-4, 1e-4, 1e-16, 1.0, use_graph=True)
+    params = abi.ts_params(1e-4, 1e-4, 1e-16, 1.0, use_graph=True, method=method)
     ref_X, ref_st = [], []
     ctxs = _contexts(wls, precision)
     for i, (c, X) in enumerate(zip(ctxs, states)):
@@ -169,7 +170,7 @@ def test_batch_ts_solve_equals_members_alone(precision):
     b = Batch(ctxs, precision=precision)
     b.ts_setup(params)
     X = _cat(states)
-    print(f"[batch-ts] {len(wls)} members precision {precision}", flush=True)
+    print(f"[batch-ts] {len(wls)} members precision {precision} method {method}", flush=True)
     stats, dts = b.ts_solve(X, [0.0] * len(wls), t_end, dt0, max_steps=8)
     got = _host(X)
     for i, wl in enumerate(wls):

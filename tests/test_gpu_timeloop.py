@@ -87,3 +87,44 @@ def test_timeloop_with_remeshing_matches_cpu(tmp_path):
             assert n > 1 and h[-1] - h[-2] < 1e-9, (path, c)
     print(f"   {len(run.outputs)} output files checked", flush=True)
     ctx.close()
+
+
+def test_reference_example_with_rk2a_matches_cpu():
+    """A configuration derived from the reference's GrainGrowth/256x256_N50
+    example, which asks for '-ts_rk_type 2a' (Heun 2(1)): the time loop
+    runs it with that method (the integrator set up from the configuration's
+    TSRK type) to the time the CPU restatement reaches in 25 accepted steps
+    from SetUpProblem; step counts, end time and next step size agree and
+    the phase fields agree to 1e-9."""
+    import torch
+    from cipher_amd.runtime import Context
+    from cipher_amd import timeloop as TL
+    wl = H.ref_case("grain_growth")
+    cfg = wl.cfg
+    assert cfg.ts_rk_type == "2a"
+    p = abi.ts_params(cfg.abstol, cfg.reltol, cfg.timestepmin, cfg.timestepmax, use_graph=True,
+                      method=abi.rk_method(cfg.ts_rk_type))
+    X0 = H.initial_state(wl)
+    _, info = H.cpu_solve(wl, X0, 0.0, 1e30, cfg.timestepmin, p, max_steps=25)
+    t_end = info["t"]
+    fc, Xc, chunks_c, rem_c = H.cpu_timeloop(cfg, wl.forest, X0, p, [t_end], None, cfg.amrinterval)
+    print(f"[timeloop 2a] {wl.ncells} cells, t_end {t_end:.6e}, cpu chunks {chunks_c}", flush=True)
+    ctx = Context(cfg)
+    ctx.upload_mesh(wl.mesh)
+    ctx.ts_setup(p)
+    X = torch.as_tensor(X0).to(ctx.device)
+    run = TL.solve(ctx, cfg, wl.forest, X, p, times=[t_end], rates=None)
+    torch.cuda.synchronize()
+    print(f"   gpu chunks {run.chunks}", flush=True)
+    assert len(run.chunks) == len(chunks_c) == 1
+    g, c = run.chunks[0], chunks_c[0]
+    assert (g[1], g[2]) == (c[1], c[2]) and c[1] >= 25
+    assert g[3] == c[3] == t_end
+    assert g[4] == pytest.approx(c[4], rel=1e-8, abs=0.0)
+    Xg = run.X.cpu().numpy()
+    L = wl.layout
+    m = H.state_mask(Xc, L, cfg)
+    assert np.array_equal(H.state_mask(Xg, L, cfg), m), "active sets differ"
+    errs = H.field_errors(np.where(m, Xg, 0.0), np.where(m, Xc, 0.0), L)
+    print("   final state errors", errs, flush=True)
+    assert errs["PF"] <= 1e-9

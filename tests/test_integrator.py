@@ -78,13 +78,17 @@ def test_controller_history_and_consecutive_rejections(lib):
     assert c2.rejected_last == 1
 
 
-def test_controller_arithmetic_matches_libm(lib):
+@pytest.mark.parametrize("method", [abi.RK_3BS, abi.RK_2A], ids=["3bs", "2a"])
+def test_controller_arithmetic_matches_libm(lib, method):
     """The controller evaluates pow and atan from their series in basic IEEE
     operations, so that the host and the device take bit-identical
     decisions; against libm's pow and atan the next step agrees to a few
-    ulps over errors from 1e-8 to 1e4, with and without history."""
+    ulps over errors from 1e-8 to 1e4, with and without history.  The
+    exponents divide by the method's order k: 3 for Bogacki-Shampine 3(2),
+    2 for Heun 2(1)."""
     from cipher_amd.runtime import ts_adapt
-    p = _params()
+    p = _params(method=method)
+    k = 3.0 if method == abi.RK_3BS else 2.0
     rng = np.random.default_rng(0)
     worst = 0.0
     for _ in range(3000):
@@ -93,9 +97,9 @@ def test_controller_arithmetic_matches_libm(lib):
         c = abi.TsCtrl()
         c.err_prev = prev
         acc, nxt = ts_adapt(p, c, 0.1, err)
-        r = (1.0 / err) ** ((0.6 if prev > 0 else 1.0) / 3.0)
+        r = (1.0 / err) ** ((0.6 if prev > 0 else 1.0) / k)
         if prev > 0:
-            r *= (1.0 / prev) ** (-0.2 / 3.0)
+            r *= (1.0 / prev) ** (-0.2 / k)
         ratio = min(max(p.safety * (1.0 + math.atan(r - 1.0)), p.clip_lo), p.clip_hi)
         ref = min(max(0.1 * ratio, p.dt_min), p.dt_max)
         assert acc == (err <= 1.0)
@@ -125,6 +129,39 @@ def test_bogacki_shampine_orders():
     r_loc = loc[1] / loc[2]
     assert 6.0 < r_est < 10.0, est
     assert 12.0 < r_loc < 20.0, loc
+
+
+def test_heun_orders():
+    """Heun 2(1) ('-ts_rk_type 2a') on the oracle RHS with frozen active
+    sets: the embedded solution is the explicit Euler step (its difference
+    from the step is O(dt^2): halving dt divides it by ~4) and the step's
+    local error is O(dt^3) (~8), which fixes the tableau; the embedded
+    solution is exactly one Euler step."""
+    wl = H.gg2d()
+    X = H.evolve(wl, H.initial_state(wl), steps=4)
+    F = oracle.rhs(wl.cfg, wl.mesh, X)
+    dt0 = 0.01 / np.abs(F).max()
+    est, loc = [], []
+    for k in range(3):
+        dt = dt0 / 2 ** k
+        Xn, Xh, _ = H.cpu_attempt(wl, X, dt, 1e-4, 1e-4, method=abi.RK_2A)
+        assert np.array_equal(Xh, X + dt * F)
+        Y = X.copy()
+        for _ in range(16):
+            Y, _, _ = H.cpu_attempt(wl, Y, dt / 16, 1e-4, 1e-4, method=abi.RK_2A)
+        est.append(np.abs(Xn - Xh).max())
+        loc.append(np.abs(Xn - Y).max())
+    r_est = est[1] / est[2]
+    r_loc = loc[1] / loc[2]
+    assert 3.0 < r_est < 5.0, est
+    assert 6.0 < r_loc < 10.0, loc
+
+
+def test_rk_method_names():
+    assert abi.rk_method("3bs") == abi.RK_3BS and abi.rk_method("2a") == abi.RK_2A
+    with pytest.raises(abi.CipherError) as e:
+        abi.rk_method("5dp")
+    assert e.value.code == abi.ERR_UNSUPPORTED
 
 
 def test_loadcases_integrate_the_temperature_rate():
@@ -171,8 +208,9 @@ def _host(t):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("method", [abi.RK_3BS, abi.RK_2A], ids=["3bs", "2a"])
 @pytest.mark.parametrize("name", ["gg2d", "ternary"])
-def test_attempt_matches_cpu(name):
+def test_attempt_matches_cpu(name, method):
     wl = getattr(H, name)()
     X = H.evolve(wl, H.initial_state(wl), steps=3)
     F = oracle.rhs(wl.cfg, wl.mesh, X)
@@ -180,12 +218,12 @@ def test_attempt_matches_cpu(name):
     Fv = F.reshape(-1, L["ndof"])
     dt = 0.02 / np.abs(Fv[:, L["pf_offset"]:L["pf_offset"] + L["pf_size"]]).max()
     dt = min(dt, 0.02 / (np.abs(Fv) / np.maximum(np.abs(X.reshape(-1, L["ndof"])), 1.0)).max())
-    p = _params()
+    p = _params(method=method)
     ctx = _ctx(wl, p)
-    print(f"[ts attempt] {name} dt={dt:.3e}", flush=True)
+    print(f"[ts attempt] {name} method {method} dt={dt:.3e}", flush=True)
     Xd = _dev(ctx, X)
     err_g = ctx.ts_attempt(Xd, dt)
-    Xn, _, err_c = H.cpu_attempt(wl, X, dt, p.atol, p.rtol)
+    Xn, _, err_c = H.cpu_attempt(wl, X, dt, p.atol, p.rtol, method=method)
     assert err_g == pytest.approx(err_c, rel=1e-8, abs=0.0)
     ctx.ts_accept(Xd)
     Xo, rc = oracle.poststep(wl.cfg, wl.mesh, Xn)
@@ -199,18 +237,21 @@ def test_attempt_matches_cpu(name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["gg2d", "gg3d_amr"])
-def test_trajectory_matches_cpu(name):
+@pytest.mark.parametrize("name,method", [("gg2d", abi.RK_3BS), ("gg3d_amr", abi.RK_3BS), ("gg2d", abi.RK_2A),
+                                         ("gg3d_amr", abi.RK_2A)], ids=["gg2d-3bs", "gg3d_amr-3bs", "gg2d-2a",
+                                                                        "gg3d_amr-2a"])
+def test_trajectory_matches_cpu(name, method):
     """Full adaptive trajectory: accepted/rejected counts, step sizes and the
-    final phase fields agree with the CPU restatement (same controller)."""
+    final phase fields agree with the CPU restatement (same controller); the
+    Heun 2(1) runs replay each step from a captured HIP graph."""
     wl = getattr(H, name)()
     X0 = H.initial_state(wl)
-    p = _params(atol=1e-3, rtol=1e-3, dt_min=1e-4, dt_max=2.0)
+    p = _params(atol=1e-3, rtol=1e-3, dt_min=1e-4, dt_max=2.0, method=method, use_graph=method == abi.RK_2A)
     t_end = 4.0 if name == "gg2d" else 1.5
     Xc, info = H.cpu_solve(wl, X0, 0.0, t_end, 1e-4, p)
     ctx = _ctx(wl, p)
     Xd = _dev(ctx, X0)
-    print(f"[ts solve] {name} cpu steps={info['steps']} rejected={info['rejected']}", flush=True)
+    print(f"[ts solve] {name} method {method} cpu steps={info['steps']} rejected={info['rejected']}", flush=True)
     st, dt_next = ctx.ts_solve(Xd, 0.0, t_end, 1e-4)
     print(f"   gpu steps={st.steps} rejected={st.rejected} t={st.t} dt={st.dt}", flush=True)
     assert (st.steps, st.rejected) == (info["steps"], info["rejected"])

@@ -80,18 +80,23 @@ def test_anisotropy_case_runs():
     assert (np.abs(r.sum(axis=1)) <= 1e-12 * (np.abs(r).max(axis=1) + 1e-300)).all()
 
 
-def test_rk_type_other_than_3bs_is_rejected_by_the_time_loop():
-    """Several reference examples ask for '-ts_rk_type 2a' in petscoptions;
-    the time loop integrates Bogacki-Shampine 3(2) only, so it refuses such a
-    configuration instead of changing its trajectory silently."""
+def test_rk_type_selects_the_integrator_method():
+    """Several reference examples ask for '-ts_rk_type 2a' in petscoptions
+    (Heun 2(1)); the time loop integrates 3bs and 2a, insists that the
+    integrator was set up with the configuration's method, and refuses other
+    types instead of changing their trajectories silently."""
     import os
     from cipher_amd import config, timeloop
     from cipher_amd import workload as W
     gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
     cfg = config.load_config(open(os.path.join(gold, "ref_grain_growth.yaml")).read())
-    assert cfg.ts_rk_type == "2a"
+    assert cfg.ts_rk_type == "2a" and abi.rk_method(cfg.ts_rk_type) == abi.RK_2A
     with pytest.raises(abi.CipherError) as e:
-        timeloop.solve(None, cfg, None, None, None)
+        timeloop.solve(None, cfg, None, None, abi.ts_params(1e-4, 1e-4, 1e-3, 1.0))
+    assert e.value.code == abi.ERR_ARG
+    cfg.ts_rk_type = "4"
+    with pytest.raises(abi.CipherError) as e:
+        timeloop.solve(None, cfg, None, None, abi.ts_params(1e-4, 1e-4, 1e-3, 1.0, method=abi.RK_2A))
     assert e.value.code == abi.ERR_UNSUPPORTED
     gen = config.load_config(W.grain_growth_yaml(res=(16, 16), ngrains=4))
-    assert gen.ts_rk_type == "3bs"
+    assert gen.ts_rk_type == "3bs" and abi.rk_method(gen.ts_rk_type) == abi.RK_3BS
