@@ -6,7 +6,10 @@ ensemble of 96 independent 2D instances (side log-uniform 64..1024 with AMR,
 10..2000 Voronoi grains, 1..4 components, 1..3 phase types, capacities
 4/8/16) run through one batch; the measured value is the batched FP32-mixed
 right-hand side, with the batched FP64 one as its control in the same line
-and the FP32-mixed error against FP64 per field.  Other workloads:
+and the FP32-mixed error against FP64 per field; whole adaptive steps of
+every member (four right-hand sides, stage combinations, error norm, the
+device step controller and PostStep, --whole-steps of them from each
+member's timestepmin) are timed separately.  Other workloads:
 --workload ternary (3D AMR ternary precipitation, 128^3, levels 4-7),
 gg3d (3D AMR grain growth, 64^3, 500 grains), gg2d (128^2 periodic, 20
 grains).
@@ -250,6 +253,71 @@ def rel_errors(A, B, L):
     return rel
 
 
+def whole_steps(args, b, ctxs, wls):
+    """Whole adaptive steps of every member (four right-hand sides, stage
+    combinations, error norm, the device step controller and PostStep),
+    each member from its SetUpProblem state and its own timestepmin (the
+    evolved states sit at the edge of their capacity), timed separately from
+    the right-hand side alone.  A member whose active sets outgrow its
+    capacity within the steps (the reference asserts there) is left out and
+    the rest run again in a batch of their own, autotuned like the first."""
+    import re
+    import torch
+    from cipher_amd import abi
+    from cipher_amd.runtime import Batch
+    X0 = [prepare_state(c, w, 0)[0] for c, w in zip(ctxs, wls)]
+    members = list(range(len(wls)))
+    left_out = []
+    out = {}
+    bw = b
+    for _ in range(8):
+        Xw = torch.cat([X0[i] for i in members])
+        if bw is not b:
+            bw.autotune(Xw, reps=2)
+        n = len(members)
+        dt0 = [wls[i].cfg.timestepmin for i in members]
+        failed = None
+        for prec in (abi.FP32_MIXED, abi.FP64):
+            key = "fp32_mixed" if prec else "fp64"
+            bw.set_precision(prec)
+            bw.ts_setup(abi.ts_params(1e-4, 1e-4, 1e-16, 1.0, use_graph=True))
+            Xs = Xw.clone()
+            try:
+                bw.ts_solve(Xs, [0.0] * n, [1e30] * n, dt0, max_steps=1)    # capture and warm-up
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                stats, _ = bw.ts_solve(Xs, [0.0] * n, [1e30] * n, dt0, max_steps=args.whole_steps)
+                torch.cuda.synchronize()
+                el = time.perf_counter() - t0
+            except abi.CipherError as e:
+                m = re.search(r"in member (\d+)", str(e))
+                if e.code != abi.ERR_CAPACITY or not m:
+                    raise
+                failed = members[int(m.group(1))]
+                break
+            steps = max(s.steps for s in stats)
+            out[key] = {"ms_per_step": 1e3 * el / max(steps, 1), "steps_per_member": steps,
+                        "attempts_max": max(s.steps + s.rejected for s in stats), "rhs_per_attempt": 4}
+            print(f"[bench] whole steps {key}: {steps} steps per member, {1e3 * el / max(steps, 1):.3f} ms per step",
+                  file=sys.stderr, flush=True)
+        if failed is None:
+            break
+        print(f"[bench] whole steps: member {failed} ({wls[failed].name}) outgrows its capacity, left out",
+              file=sys.stderr, flush=True)
+        left_out.append(failed)
+        members.remove(failed)
+        out = {}
+        if bw is not b:
+            bw.close()
+        bw = Batch([ctxs[i] for i in members])
+    if bw is not b:
+        bw.close()
+    out["members"] = len(members)
+    out["cells"] = int(sum(wls[i].ncells for i in members))
+    out["left_out"] = [wls[i].name for i in left_out]
+    return out
+
+
 def run_ensemble(args):
     """Batched ensemble: `--members` independent 2D instances (side
     log-uniform 64..1024 with AMR, 10..2000 grains, 1..4 components, 1..3
@@ -326,6 +394,7 @@ def run_ensemble(args):
             c.rhs(x, f)
     ms_seq = timed(lambda: [c.rhs(x, f) for c, x, f in views], max(1, args.steps // 4))
     print(f"[bench] one context at a time fp64: {ms_seq:.3f} ms per ensemble RHS", file=sys.stderr, flush=True)
+    whole = whole_steps(args, b, ctxs, wls)
     ms32, F32, shapes32 = res[abi.FP32_MIXED]
     ms64, F64, shapes64 = res[abi.FP64]
     A, B = F32.cpu().numpy(), F64.cpu().numpy()
@@ -425,6 +494,7 @@ def run_ensemble(args):
                          "active_flags_equal": flags_equal,
                          "phase_fraction_rate_within_1e-6": per_field.get("PF", {}).get("max", 0.0) <= 1e-6},
         "one_context_at_a_time_fp64": {"ms_per_step": ms_seq, "batching_speedup_fp64": ms_seq / ms64},
+        "whole_step": whole,
         "autotuned_shapes_fp32": shapes32,
         "cpu_baseline": cpu,
     }
@@ -459,6 +529,8 @@ def main():
     ap.add_argument("--workload", default="ensemble", help="ensemble (default), ternary, gg3d or gg2d")
     ap.add_argument("--members", type=int, default=96, help="instances of the ensemble workload")
     ap.add_argument("--evolve", type=int, default=20)
+    ap.add_argument("--whole-steps", type=int, default=10,
+                    help="adaptive steps of every ensemble member timed whole (stages, norm, controller, PostStep)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of autotuned ones")
