@@ -56,6 +56,20 @@ __device__ __forceinline__ T gsh(T v, int src) {
     else return __shfl(v, src, G);
 }
 
+// Loops over a group's slots g < n that sum shuffled values.  A group of up
+// to four lanes runs all G iterations, unrolled, with the arithmetic guarded
+// by g < n: the shuffles of every iteration (which read lanes of the group
+// only) issue together instead of one shuffle latency per iteration, and the
+// guarded sums keep the loop's order.  Wider groups loop over n.  A guarded
+// iteration g >= n must not index a table with a shuffled phase id (the id
+// of a slot beyond n is not a phase).  Only the FP64 list kernels unroll
+// (S = G); the prep kernels and the FP32-mixed list kernels keep the loops
+// over n (S = 64), which measured faster for them.
+template <int S>
+__device__ __forceinline__ constexpr int slot_end(int n) { return S <= 4 ? S : n; }
+#define CIPHER_PRAGMA(x) _Pragma(#x)
+#define CIPHER_SLOT_UNROLL_OF(S) CIPHER_PRAGMA(unroll S <= 4 ? S : 1)
+
 // interface energy of an interface type at T (src/cipher.c:425, 462-465)
 template <class R>
 __device__ __forceinline__ R itf_energy(const DevCfg &c, int it, R T) {
@@ -101,30 +115,36 @@ struct Acc {
     }
 };
 
-// per-cell interpolants of the active slots (src/utility.c:470-535)
-template <int G, class R>
+// per-cell interpolants of the active slots (src/utility.c:470-535); S the
+// slot-loop bound of slot_end
+template <int G, class R, int S = G>
 __device__ __forceinline__ void interpolant(const DevCfg &c, int n, R phi, R &hh, R &h) {
     hh = (c.interp == CIPHER_INTERP_QUADRATIC) ? phi * phi : phi * phi * (R(3) - R(2) * phi);
     R hsum = R(0);
-    #pragma unroll 1
-    for (int g = 0; g < n; g++) hsum += gsh<G>(hh, g);
+    CIPHER_SLOT_UNROLL_OF(S)
+    for (int g = 0; g < slot_end<S>(n); g++) {
+        const R v = gsh<G>(hh, g);
+        if (g < n) hsum += v;
+    }
     if (c.interp == CIPHER_INTERP_LINEAR) h = phi;
     else if (n == 1) h = R(1);
     else h = hh / hsum;
 }
 
 // in <- dh/dphi^T in for the slot-distributed vector (src/utility.c:478-557)
-template <int G, class R>
+template <int G, class R, int S = G>
 __device__ __forceinline__ R interp_deriv(const DevCfg &c, int n, R phi, R hh, R in) {
     if (c.interp == CIPHER_INTERP_LINEAR) return in;
     if (n == 1) return R(0);
     R avg = R(0), sum = R(0);
-    #pragma unroll 1
-    for (int g = 0; g < n; g++) {
+    CIPHER_SLOT_UNROLL_OF(S)
+    for (int g = 0; g < slot_end<S>(n); g++) {
         const R hg = gsh<G>(hh, g);
         const R ig = gsh<G>(in, g);
-        sum += hg;
-        avg += hg * ig;
+        if (g < n) {
+            sum += hg;
+            avg += hg * ig;
+        }
     }
     if (c.interp == CIPHER_INTERP_QUADRATIC) return R(2) * phi * (in - avg / sum) / sum;
     return R(6) * phi * (R(1) - phi) * (in - avg / sum) / sum;
@@ -197,7 +217,7 @@ __device__ __forceinline__ void prep_one(const double *x, long long cell, double
     const double T = x[c.tmo];
     const bool iso = wave_iso(c, T);
     double hh, h;
-    interpolant<G, double>(c, n, phi, hh, h);
+    interpolant<G, double, 64>(c, n, phi, hh, h);
     if (!on) h = 0.0;
     const int mat = c.phase_material[myid];
     const cipher_material &m = c.mat[mat];
@@ -218,18 +238,21 @@ __device__ __forceinline__ void prep_one(const double *x, long long cell, double
         // per-slot terms h*me and h*nu_s*y, summed in slot order
 #pragma unroll
         for (int k = 0; k < NC; k++) { comp[k] = 0.0; mob[k] = 0.0; }
-        #pragma unroll 1
-        for (int g = 0; g < n; g++) {
+        CIPHER_SLOT_UNROLL_OF(64)
+        for (int g = 0; g < slot_end<64>(n); g++) {
             const int nsg = gsh<G>(ns, g);
 #pragma unroll
-            for (int k = 0; k < NC; k++) mob[k] += gsh<G>(h * me[k], g);
+            for (int k = 0; k < NC; k++) {
+                const double v = gsh<G>(h * me[k], g);
+                if (g < n) mob[k] += v;
+            }
 #pragma unroll
             for (int s = 0; s < 2; s++) {
                 const double st = on && s < ns ? m.stoichiometry[s] : 0.0;
 #pragma unroll
                 for (int k = 0; k < NC; k++) {
                     const double v = gsh<G>(h * st * y[s * NC + k], g);
-                    if (s < nsg) comp[k] += v;
+                    if (g < n && s < nsg) comp[k] += v;
                 }
             }
         }
@@ -453,6 +476,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
     constexpr int ND = NC - 1;
     constexpr int NDV = Cnt<ND>::v;
     constexpr bool MIXED = sizeof(R) == 4;
+    constexpr int SG = MIXED ? 64 : G;                     // slot loops (slot_end): FP64 unrolls
     extern __shared__ int ptype[];                         // CPB*G*G ints
     const int CPB = blockDim.x / G;
     const int D = c.D;
@@ -811,7 +835,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
         // nearly cancel between phases, beyond FP32's reach); FP32-mixed
         // rounds them for its FP32 terms
         double hhd, hd;
-        interpolant<G, double>(c, n, phid, hhd, hd);
+        interpolant<G, double, SG>(c, n, phid, hhd, hd);
         if (!on) { hd = 0.0; hhd = 0.0; }
         const R h = (R)hd;
         const int mat = on ? c.phase_material[myid] : 0;
@@ -903,36 +927,36 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
         if (G > 1 && n > 1) {
             // capillary driving force (src/cipher.c:474-497), replayed per slot
             R cap = R(0), capabs = R(0);
-            #pragma unroll 1
-            for (int gk = 0; gk < n; gk++) {
+            // (pairs and junctions of slots beyond n read pair-table entries
+            // of the group that hold no pair: in bounds, and discarded)
+            CIPHER_SLOT_UNROLL_OF(SG)
+            for (int gk = 0; gk < slot_end<SG>(n); gk++) {
                 const R phk = gsh<G>(phi, gk), lapk = gsh<G>(plap, gk);
-                const int idk = gsh<G>(myid, gk);
-                #pragma unroll 1
-                for (int gj = gk + 1; gj < n; gj++) {
+                CIPHER_SLOT_UNROLL_OF(SG)
+                for (int gj = gk + 1; gj < slot_end<SG>(n); gj++) {
                     const R phj = gsh<G>(phi, gj), lapj = gsh<G>(plap, gj);
-                    const int idj = gsh<G>(myid, gj);
                     const R *pkj = ptab + PT * pair_index<G>(gk, gj);
                     const R e = pkj[0];
                     const R w = pkj[1];
+                    const bool pj = gj < n;
                     if constexpr (MIXED) {
                         const R iw = pkj[3], wp = pkj[4];
                         const R lapaj = gsh<G>(lapabs, gj), lapak = gsh<G>(lapabs, gk);
-                        if (slot == gk) {
+                        if (pj && slot == gk) {
                             cap -= e * (phj * iw + lapj * wp);
                             capabs += fabsf(e) * (fabsf(phj * iw) + lapaj * fabsf(wp));
                         }
-                        if (slot == gj) {
+                        if (pj && slot == gj) {
                             cap -= e * (phk * iw + lapk * wp);
                             capabs += fabsf(e) * (fabsf(phk * iw) + lapak * fabsf(wp));
                         }
                     } else {
-                        if (slot == gk) cap -= e * (phj / w + lapj * w / (R)CPI / (R)CPI);
-                        if (slot == gj) cap -= e * (phk / w + lapk * w / (R)CPI / (R)CPI);
+                        if (pj && slot == gk) cap -= e * (phj / w + lapj * w / (R)CPI / (R)CPI);
+                        if (pj && slot == gj) cap -= e * (phk / w + lapk * w / (R)CPI / (R)CPI);
                     }
-                    #pragma unroll 1
-                    for (int gi = gj + 1; gi < n; gi++) {
+                    CIPHER_SLOT_UNROLL_OF(SG)
+                    for (int gi = gj + 1; gi < slot_end<SG>(n); gi++) {
                         const R phi_i = gsh<G>(phi, gi);
-                        const int idi = gsh<G>(myid, gi);
                         const R *pki = ptab + PT * pair_index<G>(gk, gi);
                         const R *pji = ptab + PT * pair_index<G>(gj, gi);
                         R tj = R(0);
@@ -947,13 +971,13 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                         if (slot == gk) t3 = tj * phj * phi_i;
                         if (slot == gj) t3 = tj * phk * phi_i;
                         if (slot == gi) t3 = tj * phk * phj;
-                        if (slot == gk || slot == gj || slot == gi) {
+                        if (gi < n && (slot == gk || slot == gj || slot == gi)) {
                             cap -= t3;
                             if constexpr (MIXED) capabs += fabsf(t3);
                         }
                     }
                 }
-                if (slot == gk) {
+                if (gk < n && slot == gk) {
                     cap *= R(8);
                     cap += lapk;
                     if constexpr (MIXED) capabs = R(8) * capabs + lapabs;
@@ -1010,39 +1034,38 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                     }
                 }
             }
-            chs = interp_deriv<G, double>(c, n, phid, hhd, chs);
+            chs = interp_deriv<G, double, SG>(c, n, phid, hhd, chs);
             // unconstrained rate and active flags (src/cipher.c:524-539); the
             // pair rates are summed in FP64 in both modes
             double ru = 0.0;
             R ruabs = R(0);
             R nact = R(0);
-            #pragma unroll 1
-            for (int gk = 0; gk < n; gk++) {
+            CIPHER_SLOT_UNROLL_OF(SG)
+            for (int gk = 0; gk < slot_end<SG>(n); gk++) {
                 const R capk = gsh<G>(cap, gk);
                 const double chk = gsh<G>(chs, gk), hk = gsh<G>(hd, gk);
-                const int idk = gsh<G>(myid, gk);
                 R capak = R(0);
                 if constexpr (MIXED) capak = gsh<G>(capabs, gk);
-                #pragma unroll 1
-                for (int gj = gk + 1; gj < n; gj++) {
+                CIPHER_SLOT_UNROLL_OF(SG)
+                for (int gj = gk + 1; gj < slot_end<SG>(n); gj++) {
                     const R capj = gsh<G>(cap, gj);
                     const double chj = gsh<G>(chs, gj), hj = gsh<G>(hd, gj);
-                    const int idj = gsh<G>(myid, gj);
                     const R mkj = ptab[PT * pair_index<G>(gk, gj) + 2];
                     const double rin = (double)(capk - capj) - (chk - chj) * 8.0 * sqrt(hk * hj) / CPI;
                     const double rv = (double)mkj * rin;
-                    if (slot == gk) ru += rv;
-                    if (slot == gj) ru -= rv;
+                    const bool pj = gj < n;
+                    if (pj && slot == gk) ru += rv;
+                    if (pj && slot == gj) ru -= rv;
                     if constexpr (MIXED) {
                         // the chemical term is FP64 in both modes (identical
                         // bits); the FP32 errors are those of the two capillary
                         // terms (bounded by their magnitudes), of their
                         // difference and of the mobility's rounding
                         const R capaj = gsh<G>(capabs, gj);
-                        if (slot == gk || slot == gj) ruabs += fabsf(mkj) * (capak + capaj + (float)fabs(rin));
+                        if (pj && (slot == gk || slot == gj)) ruabs += fabsf(mkj) * (capak + capaj + (float)fabs(rin));
                     }
                 }
-                if (slot == gk) {
+                if (gk < n && slot == gk) {
                     act = (phid > CIPHER_TOL && phid < 1.0 - CIPHER_TOL) || (phid < CIPHER_TOL && ru > 0.0)
                           || (phid > 1.0 - CIPHER_TOL && ru < 0.0);
                     // the decision reads the sign of ru: is it inside the FP32
@@ -1059,29 +1082,30 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                     }
                 }
             }
-            #pragma unroll 1
-            for (int g = 0; g < n; g++) nact += gsh<G>((int)act, g) ? R(1) : R(0);
-            // constrained rate (src/cipher.c:541-555)
-            #pragma unroll 1
-            for (int gk = 0; gk < n; gk++) {
+            CIPHER_SLOT_UNROLL_OF(SG)
+            for (int g = 0; g < slot_end<SG>(n); g++) {
+                const int ag = gsh<G>((int)act, g);
+                if (g < n) nact += ag ? R(1) : R(0);
+            }
+            // constrained rate (src/cipher.c:541-555); a slot beyond n is
+            // never active
+            CIPHER_SLOT_UNROLL_OF(SG)
+            for (int gk = 0; gk < slot_end<SG>(n); gk++) {
                 const int ak = gsh<G>((int)act, gk);
-                if (!ak) continue;
                 const R capk = gsh<G>(cap, gk);
                 const double chk = gsh<G>(chs, gk), hk = gsh<G>(hd, gk);
-                const int idk = gsh<G>(myid, gk);
-                #pragma unroll 1
-                for (int gj = gk + 1; gj < n; gj++) {
+                CIPHER_SLOT_UNROLL_OF(SG)
+                for (int gj = gk + 1; gj < slot_end<SG>(n); gj++) {
                     const int aj = gsh<G>((int)act, gj);
                     const R capj = gsh<G>(cap, gj);
                     const double chj = gsh<G>(chs, gj), hj = gsh<G>(hd, gj);
-                    const int idj = gsh<G>(myid, gj);
-                    if (!aj) continue;
                     const R mkj = ptab[PT * pair_index<G>(gk, gj) + 2];
                     const double rv = (double)mkj * ((double)(capk - capj) - (chk - chj) * 8.0 * sqrt(hk * hj) / CPI);
-                    if (slot == gk) pfrhs += rv;
-                    if (slot == gj) pfrhs -= rv;
+                    const bool use = ak && aj && gj < n;
+                    if (use && slot == gk) pfrhs += rv;
+                    if (use && slot == gj) pfrhs -= rv;
                 }
-                if (slot == gk) pfrhs /= (double)nact;
+                if (ak && gk < n && slot == gk) pfrhs /= (double)nact;
             }
         }
         const int gbase = (threadIdx.x & 63) & ~(G - 1);
@@ -1204,8 +1228,8 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
             double dcdm[ND * ND];
 #pragma unroll
             for (int q = 0; q < ND * ND; q++) dcdm[q] = 0.0;
-            #pragma unroll 1
-            for (int g = 0; g < n; g++) {
+            CIPHER_SLOT_UNROLL_OF(SG)
+            for (int g = 0; g < slot_end<SG>(n); g++) {
                 const int nsg = gsh<G>(ns, g);
 #pragma unroll
                 for (int s = 0; s < 2; s++) {
@@ -1213,7 +1237,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
 #pragma unroll
                     for (int q = 0; q < ND * ND; q++) {
                         const double v = gsh<G>(hq * st * st * tpot[s * ND * ND + q], g);
-                        if (s < nsg) dcdm[q] += v;
+                        if (g < n && s < nsg) dcdm[q] += v;
                     }
                 }
             }
@@ -1237,9 +1261,12 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                         if (s < ns) wpf += m.stoichiometry[s] * yd[s * NC + q];
                     wpf -= wcp[q];
                     if (!on) wpf = 0.0;
-                    wpf = interp_deriv<G, double>(c, n, phid, hhq, wpf);
-                    #pragma unroll 1
-                    for (int g = 0; g < n; g++) wdp[q] -= gsh<G>(pfq * wpf, g);
+                    wpf = interp_deriv<G, double, SG>(c, n, phid, hhq, wpf);
+                    CIPHER_SLOT_UNROLL_OF(SG)
+                    for (int g = 0; g < slot_end<SG>(n); g++) {
+                        const double v = gsh<G>(pfq * wpf, g);
+                        if (g < n) wdp[q] -= v;
+                    }
                 }
             }
             // Chemsource (src/material.c:321-351), called with the interpolants
@@ -1277,8 +1304,8 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
 #pragma unroll
             for (int q = 0; q < ND; q++) wdp[q] += ((double)dlap[q].s + csrc[q]);
             // sum_g sum_s h nu (T_ex . EXdot + T_T * Tdot)
-            #pragma unroll 1
-            for (int g = 0; g < n; g++) {
+            CIPHER_SLOT_UNROLL_OF(SG)
+            for (int g = 0; g < slot_end<SG>(n); g++) {
                 const int nsg = gsh<G>(ns, g);
 #pragma unroll
                 for (int s = 0; s < 2; s++) {
@@ -1286,7 +1313,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
 #pragma unroll
                     for (int q = 0; q < ND; q++) {
                         const double v = gsh<G>(hq * st * (tv[s * ND + q] + tT[s * ND + q] * tmq), g);
-                        if (s < nsg) wdp[q] -= v;
+                        if (g < n && s < nsg) wdp[q] -= v;
                     }
                 }
             }
