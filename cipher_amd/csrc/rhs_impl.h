@@ -56,7 +56,8 @@ __device__ __forceinline__ T gsh(T v, int src) {
     else return __shfl(v, src, G);
 }
 
-// Loops over a group's slots g < n that sum shuffled values.  A group of up
+// Loops over a group's slots g < n (or a face chunk's faces t < nfc) that sum
+// shuffled values.  A group of up
 // to four lanes runs all G iterations, unrolled, with the arithmetic guarded
 // by g < n: the shuffles of every iteration (which read lanes of the group
 // only) issue together instead of one shuffle latency per iteration, and the
@@ -718,9 +719,11 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                     phn[r] = ld ? xt[c.pfo + k] : 0.0;
                 }
             }
-            #pragma unroll 1
-            for (int t = 0; t < nfc; t++) {
-                const int gt = gnext;
+            // (faces t >= nfc of an unrolled chunk do nothing)
+            CIPHER_SLOT_UNROLL_OF(SG)
+            for (int t = 0; t < slot_end<SG>(nfc); t++) {
+                const bool tv = t < nfc;
+                const int gt = tv ? gnext : 0;
                 double idc[RM], phc[RM];
 #pragma unroll
                 for (int r = 0; r < RM; r++) { idc[r] = idn[r]; phc[r] = phn[r]; }
@@ -786,11 +789,14 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
                 }
                 if constexpr (NC > 1) {
 #pragma unroll
-                    for (int q = 0; q < ND; q++) dlap[q].add(gsh<G>(cterm[q], t));
+                    for (int q = 0; q < ND; q++) {
+                        const double v = gsh<G>(cterm[q], t);
+                        if (tv) dlap[q].add(v);
+                    }
                 }
                 if (c.has_tcond) {
                     const R tt = gsh<G>(tterm, t);
-                    if (gsh<G>(tvalid, t)) tlap.add(tt);
+                    if (tv && gsh<G>(tvalid, t)) tlap.add(tt);
                 }
             }
         }
