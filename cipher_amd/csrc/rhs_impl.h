@@ -63,9 +63,8 @@ __device__ __forceinline__ T gsh(T v, int src) {
 // only) issue together instead of one shuffle latency per iteration, and the
 // guarded sums keep the loop's order.  Wider groups loop over n.  A guarded
 // iteration g >= n must not index a table with a shuffled phase id (the id
-// of a slot beyond n is not a phase).  Only the FP64 list kernels unroll
-// (S = G); the prep kernels and the FP32-mixed list kernels keep the loops
-// over n (S = 64), which measured faster for them.
+// of a slot beyond n is not a phase).  The list kernels unroll (S = G); the
+// prep kernels keep the loops over n (S = 64), which measured faster there.
 template <int S>
 __device__ __forceinline__ constexpr int slot_end(int n) { return S <= 4 ? S : n; }
 #define CIPHER_PRAGMA(x) _Pragma(#x)
@@ -477,7 +476,7 @@ __device__ __forceinline__ void rhs_cells(const RhsArgs &a, const DevCfg &c, lon
     constexpr int ND = NC - 1;
     constexpr int NDV = Cnt<ND>::v;
     constexpr bool MIXED = sizeof(R) == 4;
-    constexpr int SG = MIXED ? 64 : G;                     // slot loops (slot_end): FP64 unrolls
+    constexpr int SG = G;                                  // slot loops (slot_end)
     extern __shared__ int ptype[];                         // CPB*G*G ints
     const int CPB = blockDim.x / G;
     const int D = c.D;
